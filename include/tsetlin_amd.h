@@ -1,0 +1,188 @@
+/*
+ * tsetlin_amd.h -- C ABI of the MI355X (gfx950) Tsetlin-machine engine.
+ *
+ * Drop-in boundary for the classifier/encoder train and predict path of
+ * santoku.tsetlin (module santoku.tsetlin.capi, lib/santoku/tsetlin/capi.c).
+ * Every entry point names the reference function it replaces.  Plain C types
+ * only: pointers, sizes, a callback.  Host pointers unless the name ends in
+ * _dev (device pointers, data already resident in HBM).
+ *
+ * Error convention: every call returns TM_OK (0) or a negative TM_E* code;
+ * tm_last_error() returns a thread-local message for the last failure.  The
+ * reference raises luaL_error / tk_lua_verror instead; a Lua shim maps a
+ * non-zero status to luaL_error(L, "%s", tm_last_error()).
+ *
+ * Handles are not re-entrant: one thread at a time per handle (the reference
+ * is called from one Lua thread).  Each handle owns a HIP stream.
+ */
+#ifndef TSETLIN_AMD_H
+#define TSETLIN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* capi.c:1802 (lua_pushinteger(L, TK_CVEC_BITS); "align"). */
+#define TM_ALIGN 8
+
+/* capi.c:41-44 (tk_tsetlin_type_t); persisted as int32 at capi.c:1543. */
+#define TM_CLASSIFIER 0
+#define TM_ENCODER 1
+
+#define TM_OK 0
+#define TM_E_ARG (-1)        /* bad argument (tk_lua_verror in the reference) */
+#define TM_E_ALLOC (-2)      /* host or device allocation failed */
+#define TM_E_STATE (-3)      /* operation not valid in this state */
+#define TM_E_FORMAT (-4)     /* malformed persisted data */
+#define TM_E_SIZE (-5)       /* buffer size mismatch */
+#define TM_E_DEVICE (-6)     /* HIP runtime error / no device */
+#define TM_E_UNSUPPORTED (-7)/* valid for the reference, not built here */
+
+typedef struct tm_machine tm_machine;
+
+/* Options of create/reconfigure.  Field names are the Lua option keys of
+ * capi.c:631-660 (classifier) and :668-697 (encoder): for an encoder,
+ * `classes` carries "hidden" and `features` carries "visible". */
+typedef struct {
+  uint32_t classes;          /* required, > 0 */
+  uint32_t features;         /* required */
+  uint32_t clauses;          /* required, > 0; rounded up to a multiple of 8 */
+  uint32_t clause_tolerance; /* required, > 0 */
+  uint32_t clause_maximum;   /* required, > 0 */
+  uint32_t state;            /* "state" (state bits), default 8, >= 2 */
+  uint32_t include_bits;     /* default 1 (0 means 1) */
+  double target;             /* default -1: sqrt(clauses/2)*tolerance */
+  double negative;           /* default -1: 1/classes */
+  double specificity;        /* required, > 0 */
+  int32_t reusable;          /* default 0 */
+  int32_t individualized;    /* default 0; 1 is TM_E_UNSUPPORTED in this build */
+  uint64_t seed;             /* RNG seed of train (no counterpart: the reference
+                                seeds each OpenMP thread with its thread id) */
+} tm_options;
+
+/* Fill `o` with the reference's defaults (state 8, include_bits 1, target -1,
+ * negative -1, reusable 0, seed 0); required fields are zeroed. */
+void tm_options_default(tm_options *o);
+
+/* Epoch callback: the `each` option of train (capi.c:1168-1182).  Called on the
+ * calling thread after each epoch with the 1-based epoch number, after the
+ * epoch's actions are visible to tm_predict*.  Return 1 to continue, 0 to stop
+ * (the Lua callback returning false), negative to report an error (printed
+ * to stderr; training stops, and train still returns TM_OK, as the reference
+ * does when lua_pcall fails). */
+typedef int (*tm_epoch_fn)(void *user, uint32_t epoch);
+
+/* create("classifier", opts) -> tk_tsetlin_create_classifier, capi.c:627-660,
+ * init at capi.c:449-506. */
+int tm_create_classifier(const tm_options *opts, tm_machine **out);
+
+/* create("encoder", opts) -> tk_tsetlin_create_encoder, capi.c:662-697,
+ * init at capi.c:508-526. */
+int tm_create_encoder(const tm_options *opts, tm_machine **out);
+
+/* machine:train{samples, problems, solutions, iterations, each}
+ * -> tk_tsetlin_train_classifier, capi.c:1112-1191.
+ * problems: n rows of input_chunks bytes (flip-interleaved literals);
+ * solutions: n int64 class labels. */
+int tm_train_classifier(tm_machine *m, uint32_t n, const uint8_t *problems, const int64_t *solutions,
+                        uint32_t iterations, tm_epoch_fn each, void *user);
+
+/* machine:train{samples, sentences, codes, iterations, each}
+ * -> tk_tsetlin_train_encoder, capi.c:1193-1282.
+ * codes: n rows of ceil(hidden/8) bytes, bit h of a row is the target of
+ * hidden bit h. */
+int tm_train_encoder(tm_machine *m, uint32_t n, const uint8_t *sentences, const uint8_t *codes,
+                     uint32_t iterations, tm_epoch_fn each, void *user);
+
+/* Same as above with problems/solutions/codes already in device memory
+ * (same byte layouts).  No counterpart in the reference (its data is host
+ * memory); for callers that keep datasets resident in HBM. */
+int tm_train_classifier_dev(tm_machine *m, uint32_t n, const uint8_t *problems_dev, const int64_t *solutions_dev,
+                            uint32_t iterations, tm_epoch_fn each, void *user);
+int tm_train_encoder_dev(tm_machine *m, uint32_t n, const uint8_t *sentences_dev, const uint8_t *codes_dev,
+                         uint32_t iterations, tm_epoch_fn each, void *user);
+
+/* machine:predict(problems, n) for a classifier -> tk_tsetlin_predict_classifier,
+ * capi.c:903-948.  out: n int64 class ids. */
+int tm_predict_classifier(tm_machine *m, const uint8_t *problems, uint32_t n, int64_t *out);
+
+/* machine:predict(problems, n) for an encoder -> tk_tsetlin_predict_encoder,
+ * capi.c:950-995.  out: n * ceil(hidden/8) bytes (padding bits zero). */
+int tm_predict_encoder(tm_machine *m, const uint8_t *problems, uint32_t n, uint8_t *out);
+
+/* Device-pointer variants of predict (inputs and outputs in device memory). */
+int tm_predict_classifier_dev(tm_machine *m, const uint8_t *problems_dev, uint32_t n, int64_t *out_dev);
+int tm_predict_encoder_dev(tm_machine *m, const uint8_t *problems_dev, uint32_t n, uint8_t *out_dev);
+
+/* machine:persist(nil) -> tk_tsetlin_persist, capi.c:1538-1562 (layout
+ * capi.c:1509-1531).  Writes the persisted bytes into `buf` when cap is large
+ * enough; *len always receives the required size.  TM_E_SIZE if cap < *len. */
+int tm_persist(tm_machine *m, uint8_t *buf, size_t cap, size_t *len);
+
+/* machine:persist(path) -> same, to a file. */
+int tm_persist_file(tm_machine *m, const char *path);
+
+/* tsetlin.load(data, true) -> tk_tsetlin_load, capi.c:1751-1772 (reader
+ * capi.c:1675-1737).  The loaded machine has no automata state: train on it is
+ * TM_E_STATE (the reference dereferences NULL state there). */
+int tm_load(const uint8_t *data, size_t len, tm_machine **out);
+
+/* tsetlin.load(path) -> same, from a file. */
+int tm_load_file(const char *path, tm_machine **out);
+
+/* machine:checkpoint(cvec) -> tk_tsetlin_checkpoint, capi.c:1564-1576: copy
+ * the actions (action_chunks bytes, reference layout) into buf. *len gets the
+ * size; TM_E_SIZE if cap is too small. */
+int tm_checkpoint(tm_machine *m, uint8_t *buf, size_t cap, size_t *len);
+
+/* machine:restore(cvec) -> tk_tsetlin_restore, capi.c:1578-1588: TM_E_SIZE
+ * unless len == action_chunks. */
+int tm_restore(tm_machine *m, const uint8_t *buf, size_t len);
+
+/* machine:reconfigure(opts) -> tk_tsetlin_reconfigure, capi.c:1590-1673.
+ * Reads clauses, clause_tolerance, clause_maximum, include_bits, target,
+ * specificity (and seed) from opts; TM_E_STATE unless created reusable. */
+int tm_reconfigure(tm_machine *m, const tm_options *opts);
+
+/* machine:type() -> tk_tsetlin_type, capi.c:1774-1788: TM_CLASSIFIER or
+ * TM_ENCODER, or a negative error code. */
+int tm_type(const tm_machine *m);
+
+/* machine:destroy() / __gc -> tk_tsetlin_destroy, capi.c:743-750.  Frees the
+ * handle; NULL is ignored. */
+void tm_destroy(tm_machine *m);
+
+/* Derived sizes (the persisted header fields, capi.c:1511-1525). */
+typedef struct {
+  uint32_t type, classes, features, clauses, clause_tolerance, clause_maximum, target, state_bits,
+           include_bits, input_bits, input_chunks, clause_chunks, state_chunks, action_chunks, class_chunks;
+  double specificity, negative;
+  uint8_t tail_mask, reusable, has_state, trained;
+} tm_info;
+int tm_get_info(const tm_machine *m, tm_info *out);
+
+/* Thread-local message of the last failing call on this thread. */
+const char *tm_last_error(void);
+
+/* Timing of the last train/predict call, measured with HIP events on the
+ * handle's stream: milliseconds inside train kernels (all epochs), and inside
+ * the predict kernels. */
+int tm_last_timing(const tm_machine *m, double *train_ms, double *predict_ms);
+
+/* Work counters of the last train call, summed over its epochs (no reference
+ * counterpart; used to price the kernels against the VALU roofline):
+ * [0] chunk-steps evaluated (negative-sampling skips excluded),
+ * [1] Type I events on a true clause, [2] of which grew the clause
+ * (literals <= clause_maximum), [3] Type II events on a true clause,
+ * [4] Type I events on a false clause, [5] random-literal draws. */
+#define TM_NSTATS 6
+int tm_last_train_stats(const tm_machine *m, uint64_t out[TM_NSTATS]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,784 @@
+// tm_api.cpp -- host side of the C ABI declared in include/tsetlin_amd.h.
+//
+// Mirrors the machine object of lib/santoku/tsetlin/capi.c (tk_tsetlin_t,
+// capi.c:46-83): option checks and derived sizes, the persisted byte layout,
+// checkpoint/restore/reconfigure semantics.  All automata state lives in device
+// memory; the kernels are in tm_kernels.hip.
+#include "tsetlin_amd.h"
+#include "tm_internal.h"
+#include "tm_rng.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                \
+  do {                                                                               \
+    hipError_t e_ = (hipError_t)(expr);                                              \
+    if (e_ != hipSuccess) return fail(TM_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct dbuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes)
+  {
+    if (bytes <= cap && p) return TM_OK;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = bytes ? bytes : 4;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(TM_E_ALLOC, "device allocation of %zu bytes failed: %s", want, hipGetErrorString(e));
+    }
+    cap = want;
+    return TM_OK;
+  }
+  void release()
+  {
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <typename T> T *as() const { return (T *) p; }
+};
+
+} // namespace
+
+struct tm_machine {
+  int type = TM_CLASSIFIER;
+  bool has_state = false, trained = false, reusable = false, individualized = false;
+  uint32_t classes = 0, features = 0, clauses = 0, clause_tolerance = 0, clause_maximum = 0, target = 0;
+  uint32_t state_bits = 8, include_bits = 1, input_bits = 0, input_chunks = 0, clause_chunks = 0;
+  uint32_t class_chunks = 0, state_chunks = 0, action_chunks = 0;
+  uint8_t tail_mask = 0xFF;
+  double negative = 0, specificity = 0;
+  uint64_t seed = 0;
+
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint32_t wd = 0;                  // u32 words per literal row
+  dbuf counts, actions;             // automata state
+  dbuf actT, lit;                   // predict-side views of the include plane
+  bool predict_ready = false;
+  dbuf t_rows, t_words, t_labels, t_codes, t_perm, t_perm_scratch, t_ekeys;   // training data
+  dbuf p_rows, p_words, p_sums, p_out;                                        // predict data
+  dbuf t_stats;
+  uint64_t stats[TM_NSTATS] = { 0 };
+  double train_ms = 0, predict_ms = 0;
+
+  uint64_t nclauses() const { return (uint64_t) classes * clauses; }
+  uint32_t last_eval_mask() const
+  {
+    // literal bits < input_bits inside the last word
+    uint32_t bits = input_bits - 32 * (wd - 1);
+    return bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+  }
+  uint32_t last_live_mask() const
+  {
+    // Bytes < input_chunks are live.  The last byte keeps only its tail bits
+    // when the reference's masked byte path handles it (input_chunks % 16 != 0,
+    // automata.h:97,106); inside the unmasked 128-bit bulk region all 8 bits
+    // of the byte evolve (automata.h:164-194).
+    uint32_t m = 0;
+    for (uint32_t b = 0; b < 4; b++) {
+      uint32_t byte = 4 * (wd - 1) + b;
+      if (byte >= input_chunks) break;
+      uint8_t bm = 0xFF;
+      if (byte == input_chunks - 1 && (input_chunks % 16) != 0) bm = tail_mask;
+      m |= (uint32_t) bm << (8 * b);
+    }
+    return m;
+  }
+};
+
+namespace {
+
+int check_handle(const tm_machine *m)
+{
+  if (!m) return fail(TM_E_ARG, "machine is NULL");
+  return TM_OK;
+}
+
+int use_device(tm_machine *m)
+{
+  HIP_TRY(hipSetDevice(m->device));
+  return TM_OK;
+}
+
+// capi.c:473-493: derived sizes and defaults.  The target keeps the reference's
+// float fmaxf and truncating conversion to unsigned int (capi.c:480-483).
+uint32_t derive_target(double targetf, uint32_t clauses, uint32_t tolerance)
+{
+  double t;
+  if (targetf < 0)
+    t = sqrt((double) clauses / 2.0) * (double) tolerance;
+  else
+    t = ceil(targetf >= 1 ? targetf : fmaxf(1.0f, (float)((double) clauses * targetf)));
+  return (uint32_t) t;
+}
+
+int validate_opts(const tm_options *o, const char *what)
+{
+  if (!o) return fail(TM_E_ARG, "%s: options are NULL", what);
+  // capi.c:463-472
+  if (!o->classes) return fail(TM_E_ARG, "%s: classes must be greater than 1", what);
+  if (!o->clauses) return fail(TM_E_ARG, "%s: clauses must be greater than 0", what);
+  if (!o->clause_tolerance) return fail(TM_E_ARG, "%s: clause_tolerance must be greater than 0", what);
+  if (!o->clause_maximum) return fail(TM_E_ARG, "%s: clause_maximum must be greater than 0", what);
+  if (o->state < 2) return fail(TM_E_ARG, "%s: bits must be greater than 1", what);
+  if (!(o->specificity > 0)) return fail(TM_E_ARG, "%s: specificity must be a positive number", what);
+  if (!o->features) return fail(TM_E_ARG, "%s: features must be greater than 0", what);
+  if (o->individualized) return fail(TM_E_UNSUPPORTED, "%s: individualized machines are not built in this engine", what);
+  if (o->state != 8) return fail(TM_E_UNSUPPORTED, "%s: state bits other than 8 are not built in this engine", what);
+  return TM_OK;
+}
+
+int size_check(uint64_t classes, uint64_t clauses, uint64_t input_chunks, uint64_t state_bits)
+{
+  uint64_t act = classes * clauses * input_chunks;
+  uint64_t sta = act * (state_bits - 1);
+  if (act > 0xFFFFFFFFull || sta > 0xFFFFFFFFull)
+    return fail(TM_E_ARG, "machine too large for the persisted 32-bit size fields");
+  if ((uint64_t) clauses * 64ull > 0x7FFFFFFFull)
+    return fail(TM_E_ARG, "clauses per class too large for 32-bit class sums");
+  uint32_t lanes, wpl;
+  if (tm_train_shape((uint32_t)((input_chunks + 3) / 4), &lanes, &wpl))
+    return fail(TM_E_UNSUPPORTED, "rows of %llu bytes exceed the largest train tile (4096 bytes)",
+                (unsigned long long) input_chunks);
+  return TM_OK;
+}
+
+int alloc_device(tm_machine *m)
+{
+  int rc;
+  if ((rc = m->actions.ensure(m->nclauses() * m->wd * 4))) return rc;
+  if ((rc = m->counts.ensure(m->nclauses() * (m->state_bits - 1) * m->wd * 4))) return rc;
+  HIP_TRY(tm_launch_zero(m->actions.p, m->nclauses() * m->wd * 4, m->stream));
+  m->has_state = true;
+  return TM_OK;
+}
+
+int init_runtime(tm_machine *m)
+{
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(TM_E_DEVICE, "no HIP device available");
+  HIP_TRY(hipGetDevice(&m->device));
+  HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&m->ev0));
+  HIP_TRY(hipEventCreate(&m->ev1));
+  return TM_OK;
+}
+
+// tk_tsetlin_init_classifier, capi.c:449-506.
+int create_common(const tm_options *o, int type, tm_machine **out)
+{
+  const char *what = type == TM_CLASSIFIER ? "create classifier" : "create encoder";
+  if (!out) return fail(TM_E_ARG, "%s: out is NULL", what);
+  *out = nullptr;
+  int rc = validate_opts(o, what);
+  if (rc) return rc;
+  tm_machine *m = new tm_machine();
+  m->type = type;
+  m->negative = o->negative < 0 ? 1.0 / (double) o->classes : o->negative;
+  m->classes = o->classes;
+  m->class_chunks = (o->classes + 7) / 8;
+  m->clauses = ((o->clauses + 7) / 8) * 8;
+  m->clause_tolerance = o->clause_tolerance;
+  m->clause_maximum = o->clause_maximum;
+  m->target = derive_target(o->target, m->clauses, o->clause_tolerance);
+  m->features = o->features;
+  m->state_bits = o->state;
+  m->include_bits = o->include_bits ? o->include_bits : 1;
+  m->input_bits = 2 * o->features;
+  uint32_t tail_bits = m->input_bits & 7;
+  m->tail_mask = tail_bits ? (uint8_t)((1u << tail_bits) - 1) : 0xFF;
+  m->input_chunks = (m->input_bits + 7) / 8;
+  m->clause_chunks = m->clauses / 8;
+  if ((rc = size_check(m->classes, m->clauses, m->input_chunks, m->state_bits))) { delete m; return rc; }
+  m->state_chunks = m->classes * m->clauses * (m->state_bits - 1) * m->input_chunks;
+  m->action_chunks = m->classes * m->clauses * m->input_chunks;
+  m->specificity = o->specificity;
+  m->reusable = o->reusable != 0;
+  m->seed = o->seed;
+  m->wd = (m->input_chunks + 3) / 4;
+  if ((rc = init_runtime(m)) || (rc = alloc_device(m))) { tm_destroy(m); return rc; }
+  *out = m;
+  return TM_OK;
+}
+
+// Stage host or device rows (n x input_chunks bytes) as u32 words [n][wd].
+int stage_rows(tm_machine *m, const uint8_t *rows, uint32_t n, bool dev, dbuf &bytes_buf, dbuf &words_buf,
+               const uint32_t **words)
+{
+  const size_t B = m->input_chunks, nbytes = (size_t) n * B;
+  const bool aligned = (B % 4 == 0);
+  if (dev) {
+    if (aligned && ((uintptr_t) rows % 4 == 0)) { *words = (const uint32_t *) rows; return TM_OK; }
+    int rc = words_buf.ensure((size_t) n * m->wd * 4);
+    if (rc) return rc;
+    HIP_TRY(tm_launch_repack(rows, n, (uint32_t) B, m->wd, words_buf.as<uint32_t>(), m->stream));
+    *words = words_buf.as<uint32_t>();
+    return TM_OK;
+  }
+  int rc = bytes_buf.ensure(nbytes);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(bytes_buf.p, rows, nbytes, hipMemcpyHostToDevice, m->stream));
+  if (aligned) { *words = bytes_buf.as<uint32_t>(); return TM_OK; }
+  if ((rc = words_buf.ensure((size_t) n * m->wd * 4))) return rc;
+  HIP_TRY(tm_launch_repack(bytes_buf.as<uint8_t>(), n, (uint32_t) B, m->wd, words_buf.as<uint32_t>(), m->stream));
+  *words = words_buf.as<uint32_t>();
+  return TM_OK;
+}
+
+int prepare_predict(tm_machine *m)
+{
+  if (m->predict_ready) return TM_OK;
+  int rc;
+  if ((rc = m->actT.ensure(m->nclauses() * m->wd * 4))) return rc;
+  if ((rc = m->lit.ensure(m->nclauses() * 4))) return rc;
+  HIP_TRY(tm_launch_prepare_predict(m->actions.as<uint32_t>(), m->actT.as<uint32_t>(), m->lit.as<uint32_t>(),
+                                    (uint32_t) m->nclauses(), m->wd, m->last_eval_mask(), m->stream));
+  m->predict_ready = true;
+  return TM_OK;
+}
+
+int predict_sums(tm_machine *m, const uint8_t *rows, uint32_t n, bool dev)
+{
+  int rc;
+  if ((rc = use_device(m))) return rc;
+  if ((rc = prepare_predict(m))) return rc;
+  const uint32_t *words;
+  if ((rc = stage_rows(m, rows, n, dev, m->p_rows, m->p_words, &words))) return rc;
+  if ((rc = m->p_sums.ensure((size_t) n * m->classes * 4))) return rc;
+  HIP_TRY(tm_launch_zero(m->p_sums.p, (size_t) n * m->classes * 4, m->stream));
+  tm_predict_launch L;
+  L.actT = m->actT.as<uint32_t>();
+  L.lit = m->lit.as<uint32_t>();
+  L.rows = words;
+  L.sums = m->p_sums.as<int32_t>();
+  L.n = n;
+  L.wd = m->wd;
+  L.clauses_total = (uint32_t) m->nclauses();
+  L.clauses = m->clauses;
+  L.classes = m->classes;
+  L.tolerance = m->clause_tolerance;
+  L.last_eval = m->last_eval_mask();
+  HIP_TRY(hipEventRecord(m->ev0, m->stream));
+  HIP_TRY(tm_launch_predict(L, m->stream));
+  HIP_TRY(hipEventRecord(m->ev1, m->stream));
+  return TM_OK;
+}
+
+int predict_finish_timing(tm_machine *m)
+{
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+  m->predict_ms = ms;
+  return TM_OK;
+}
+
+int predict_impl(tm_machine *m, const uint8_t *rows, uint32_t n, void *out, bool dev, int want_type)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (m->type != want_type) return fail(TM_E_ARG, "predict: machine is not a%s", want_type == TM_CLASSIFIER ? " classifier" : "n encoder");
+  if (n == 0) return TM_OK;
+  if (!rows || !out) return fail(TM_E_ARG, "predict: problems and out must not be NULL");
+  if ((rc = predict_sums(m, rows, n, dev))) return rc;
+  const size_t out_bytes = want_type == TM_CLASSIFIER ? (size_t) n * 8 : (size_t) n * m->class_chunks;
+  void *dout = out;
+  if (!dev) {
+    if ((rc = m->p_out.ensure(out_bytes))) return rc;
+    dout = m->p_out.p;
+  }
+  if (want_type == TM_CLASSIFIER)
+    HIP_TRY(tm_launch_argmax(m->p_sums.as<int32_t>(), n, m->classes, (int64_t *) dout, m->stream));
+  else
+    HIP_TRY(tm_launch_encode(m->p_sums.as<int32_t>(), n, m->classes, (uint8_t *) dout, m->stream));
+  if (!dev) HIP_TRY(hipMemcpyAsync(out, dout, out_bytes, hipMemcpyDeviceToHost, m->stream));
+  return predict_finish_timing(m);
+}
+
+// tk_tsetlin_train_classifier / _encoder, capi.c:1112-1282.
+int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *labels, const uint8_t *codes,
+               uint32_t iterations, tm_epoch_fn each, void *user, bool dev, int want_type)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (m->type != want_type) return fail(TM_E_ARG, "train: machine is not a%s", want_type == TM_CLASSIFIER ? " classifier" : "n encoder");
+  if (!m->has_state)
+    return fail(TM_E_STATE, "can't train a model loaded without state (or released after training; create with reusable=true to train again)");
+  if (n > 0 && !rows) return fail(TM_E_ARG, "train: problems must not be NULL");
+  if (n > 0 && want_type == TM_CLASSIFIER && !labels) return fail(TM_E_ARG, "train: solutions must not be NULL");
+  if (n > 0 && want_type == TM_ENCODER && !codes) return fail(TM_E_ARG, "train: codes must not be NULL");
+  if ((rc = use_device(m))) return rc;
+  m->train_ms = 0;
+  memset(m->stats, 0, sizeof m->stats);
+  m->predict_ready = false;
+  if (iterations == 0 || n == 0) {
+    // setup still runs (capi.c:1136-1146) even when no sample is visited
+    HIP_TRY(tm_launch_setup(m->counts.as<uint32_t>(), m->actions.as<uint32_t>(), m->nclauses(),
+                            m->state_bits - 1, m->wd, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    for (uint32_t e = 0; e < iterations; e++) {
+      if (!each) continue;
+      int r = each(user, e + 1);
+      if (r < 0) { fprintf(stderr, "Error in epoch callback (status %d)\n", r); break; }
+      if (r == 0) break;
+    }
+  } else {
+    const uint32_t *words;
+    if ((rc = stage_rows(m, rows, n, dev, m->t_rows, m->t_words, &words))) return rc;
+    const int64_t *dlabels = nullptr;
+    const uint8_t *dcodes = nullptr;
+    if (want_type == TM_CLASSIFIER) {
+      if (dev) dlabels = labels;
+      else {
+        if ((rc = m->t_labels.ensure((size_t) n * 8))) return rc;
+        HIP_TRY(hipMemcpyAsync(m->t_labels.p, labels, (size_t) n * 8, hipMemcpyHostToDevice, m->stream));
+        dlabels = m->t_labels.as<int64_t>();
+      }
+    } else {
+      if (dev) dcodes = codes;
+      else {
+        if ((rc = m->t_codes.ensure((size_t) n * m->class_chunks))) return rc;
+        HIP_TRY(hipMemcpyAsync(m->t_codes.p, codes, (size_t) n * m->class_chunks, hipMemcpyHostToDevice, m->stream));
+        dcodes = m->t_codes.as<uint8_t>();
+      }
+    }
+    // Without a callback, epochs run in groups inside one launch each (the
+    // chunk state stays in registers across the epochs of a group).
+    const uint32_t group = each ? 1u : 8u;
+    if ((rc = m->t_perm.ensure((size_t) group * n * 4))) return rc;
+    if ((rc = m->t_perm_scratch.ensure(tm_perm_scratch_bytes(n)))) return rc;
+    if ((rc = m->t_ekeys.ensure((size_t) group * 16))) return rc;
+    if ((rc = m->t_stats.ensure(TM_NSTATS * 8))) return rc;
+    HIP_TRY(tm_launch_zero(m->t_stats.p, TM_NSTATS * 8, m->stream));
+    tm_train_launch L;
+    memset(&L, 0, sizeof L);
+    L.counts = m->counts.as<uint32_t>();
+    L.actions = m->actions.as<uint32_t>();
+    L.rows = words;
+    L.labels = dlabels;
+    L.codes = dcodes;
+    L.perm = m->t_perm.as<uint32_t>();
+    L.ekeys = m->t_ekeys.as<uint32_t>();
+    L.n = n;
+    L.wd = m->wd;
+    L.clause_chunks = m->clause_chunks;
+    L.class_chunks = m->class_chunks;
+    L.total_chunks = m->clause_chunks * m->classes;
+    L.tolerance = m->clause_tolerance;
+    L.maximum = m->clause_maximum;
+    L.target = m->target;
+    L.input_bits = m->input_bits;
+    // capi.c:185: unsigned s = (2 * features) / specificity
+    L.s_draws = (uint32_t)((double)(2u * m->features) / m->specificity);
+    L.include_bits = m->include_bits;
+    L.state_bits = m->state_bits;
+    double q = 1.0 - m->negative;
+    L.skip_threshold = !(q > 0) ? 0 : (q >= 1 ? (1ull << 32) : (uint64_t) floor(q * 4294967296.0));
+    L.seed_lo = (uint32_t) m->seed;
+    L.seed_hi = (uint32_t)(m->seed >> 32);
+    L.encoder = want_type == TM_ENCODER;
+    L.last_live = m->last_live_mask();
+    L.last_eval = m->last_eval_mask();
+    L.stats = m->t_stats.as<unsigned long long>();
+    std::vector<uint32_t> ek(4 * group);
+    bool stop = false;
+    for (uint32_t e0 = 0; e0 < iterations && !stop; e0 += group) {
+      const uint32_t cnt = (iterations - e0) < group ? (iterations - e0) : group;
+      for (uint32_t e = 0; e < cnt; e++) {
+        tm_u4 k = tm_philox(e0 + e, 0, 0, TM_TAG_EPOCH, L.seed_lo, L.seed_hi);
+        ek[4 * e] = k.x; ek[4 * e + 1] = k.y; ek[4 * e + 2] = k.z; ek[4 * e + 3] = k.w;
+      }
+      HIP_TRY(hipMemcpyAsync(m->t_ekeys.p, ek.data(), (size_t) cnt * 16, hipMemcpyHostToDevice, m->stream));
+      HIP_TRY(tm_launch_perm(m->seed, e0, cnt, n, m->t_perm.as<uint32_t>(), m->t_perm_scratch.p,
+                             m->t_perm_scratch.cap, m->stream));
+      L.epoch_first = e0;
+      L.epochs = cnt;
+      L.init = e0 == 0;
+      HIP_TRY(hipEventRecord(m->ev0, m->stream));
+      HIP_TRY(tm_launch_train(L, m->stream));
+      HIP_TRY(hipEventRecord(m->ev1, m->stream));
+      HIP_TRY(hipStreamSynchronize(m->stream));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+      m->train_ms += ms;
+      m->predict_ready = false;
+      if (each) {
+        m->trained = true;
+        int r = each(user, e0 + 1);
+        if (r < 0) { fprintf(stderr, "Error in epoch callback (status %d)\n", r); stop = true; }
+        else if (r == 0) stop = true;
+        if ((rc = use_device(m))) return rc;
+      }
+    }
+  }
+  if (m->t_stats.p && iterations > 0 && n > 0) {
+    HIP_TRY(hipMemcpyAsync(m->stats, m->t_stats.p, sizeof m->stats, hipMemcpyDeviceToHost, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+  }
+  // capi.c:1187-1189: a non-reusable machine releases its counters
+  if (!m->reusable) {
+    m->counts.release();
+    m->has_state = false;
+  }
+  m->trained = true;
+  return TM_OK;
+}
+
+void put_u32(std::vector<uint8_t> &v, uint32_t x) { v.insert(v.end(), (uint8_t *) &x, (uint8_t *) &x + 4); }
+
+int actions_to_host(tm_machine *m, uint8_t *dst)
+{
+  // device [clause][wd] words -> reference layout [clause][input_chunks] bytes
+  const uint64_t nc = m->nclauses();
+  const size_t B = m->input_chunks;
+  if (B == (size_t) m->wd * 4) {
+    HIP_TRY(hipMemcpyAsync(dst, m->actions.p, nc * B, hipMemcpyDeviceToHost, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    return TM_OK;
+  }
+  std::vector<uint32_t> tmp(nc * m->wd);
+  HIP_TRY(hipMemcpyAsync(tmp.data(), m->actions.p, tmp.size() * 4, hipMemcpyDeviceToHost, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  for (uint64_t c = 0; c < nc; c++) memcpy(dst + c * B, &tmp[c * m->wd], B);
+  return TM_OK;
+}
+
+int actions_from_host(tm_machine *m, const uint8_t *src)
+{
+  const uint64_t nc = m->nclauses();
+  const size_t B = m->input_chunks;
+  if (B == (size_t) m->wd * 4) {
+    HIP_TRY(hipMemcpyAsync(m->actions.p, src, nc * B, hipMemcpyHostToDevice, m->stream));
+  } else {
+    std::vector<uint32_t> tmp(nc * m->wd, 0);
+    for (uint64_t c = 0; c < nc; c++) memcpy(&tmp[c * m->wd], src + c * B, B);
+    HIP_TRY(hipMemcpyAsync(m->actions.p, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice, m->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  m->predict_ready = false;
+  return TM_OK;
+}
+
+// capi.c:1509-1531 (+ the int32 type written first at capi.c:1543).
+int serialize(tm_machine *m, std::vector<uint8_t> &v)
+{
+  v.clear();
+  int32_t type = m->type;
+  v.insert(v.end(), (uint8_t *) &type, (uint8_t *) &type + 4);
+  put_u32(v, m->classes);
+  put_u32(v, m->features);
+  put_u32(v, m->clauses);
+  put_u32(v, m->clause_tolerance);
+  put_u32(v, m->clause_maximum);
+  put_u32(v, m->target);
+  put_u32(v, m->state_bits);
+  put_u32(v, m->include_bits);
+  put_u32(v, m->input_bits);
+  put_u32(v, m->input_chunks);
+  put_u32(v, m->clause_chunks);
+  put_u32(v, m->state_chunks);
+  put_u32(v, m->action_chunks);
+  v.insert(v.end(), (uint8_t *) &m->specificity, (uint8_t *) &m->specificity + 8);
+  v.push_back(m->tail_mask);
+  size_t off = v.size();
+  v.resize(off + m->action_chunks);
+  int rc = actions_to_host(m, v.data() + off);
+  if (rc) return rc;
+  v.push_back(m->individualized ? 1 : 0);
+  return TM_OK;
+}
+
+struct reader {
+  const uint8_t *p;
+  size_t len, off = 0;
+  bool take(void *dst, size_t n)
+  {
+    if (off + n > len) return false;
+    memcpy(dst, p + off, n);
+    off += n;
+    return true;
+  }
+};
+
+} // namespace
+
+extern "C" {
+
+void tm_options_default(tm_options *o)
+{
+  if (!o) return;
+  memset(o, 0, sizeof *o);
+  o->state = 8;
+  o->include_bits = 1;
+  o->target = -1.0;
+  o->negative = -1.0;
+}
+
+const char *tm_last_error(void) { return g_err; }
+
+int tm_create_classifier(const tm_options *opts, tm_machine **out) { return create_common(opts, TM_CLASSIFIER, out); }
+int tm_create_encoder(const tm_options *opts, tm_machine **out) { return create_common(opts, TM_ENCODER, out); }
+
+int tm_train_classifier(tm_machine *m, uint32_t n, const uint8_t *problems, const int64_t *solutions,
+                        uint32_t iterations, tm_epoch_fn each, void *user)
+{
+  return train_impl(m, n, problems, solutions, nullptr, iterations, each, user, false, TM_CLASSIFIER);
+}
+
+int tm_train_encoder(tm_machine *m, uint32_t n, const uint8_t *sentences, const uint8_t *codes,
+                     uint32_t iterations, tm_epoch_fn each, void *user)
+{
+  return train_impl(m, n, sentences, nullptr, codes, iterations, each, user, false, TM_ENCODER);
+}
+
+int tm_train_classifier_dev(tm_machine *m, uint32_t n, const uint8_t *problems, const int64_t *solutions,
+                            uint32_t iterations, tm_epoch_fn each, void *user)
+{
+  return train_impl(m, n, problems, solutions, nullptr, iterations, each, user, true, TM_CLASSIFIER);
+}
+
+int tm_train_encoder_dev(tm_machine *m, uint32_t n, const uint8_t *sentences, const uint8_t *codes,
+                         uint32_t iterations, tm_epoch_fn each, void *user)
+{
+  return train_impl(m, n, sentences, nullptr, codes, iterations, each, user, true, TM_ENCODER);
+}
+
+int tm_predict_classifier(tm_machine *m, const uint8_t *problems, uint32_t n, int64_t *out)
+{
+  return predict_impl(m, problems, n, out, false, TM_CLASSIFIER);
+}
+
+int tm_predict_encoder(tm_machine *m, const uint8_t *problems, uint32_t n, uint8_t *out)
+{
+  return predict_impl(m, problems, n, out, false, TM_ENCODER);
+}
+
+int tm_predict_classifier_dev(tm_machine *m, const uint8_t *problems, uint32_t n, int64_t *out)
+{
+  return predict_impl(m, problems, n, out, true, TM_CLASSIFIER);
+}
+
+int tm_predict_encoder_dev(tm_machine *m, const uint8_t *problems, uint32_t n, uint8_t *out)
+{
+  return predict_impl(m, problems, n, out, true, TM_ENCODER);
+}
+
+int tm_persist(tm_machine *m, uint8_t *buf, size_t cap, size_t *len)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (!len) return fail(TM_E_ARG, "persist: len is NULL");
+  if ((rc = use_device(m))) return rc;
+  std::vector<uint8_t> v;
+  if ((rc = serialize(m, v))) return rc;
+  *len = v.size();
+  if (!buf || cap < v.size()) return fail(TM_E_SIZE, "persist: buffer of %zu bytes, need %zu", cap, v.size());
+  memcpy(buf, v.data(), v.size());
+  return TM_OK;
+}
+
+int tm_persist_file(tm_machine *m, const char *path)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (!path) return fail(TM_E_ARG, "persist: path is NULL");
+  if ((rc = use_device(m))) return rc;
+  std::vector<uint8_t> v;
+  if ((rc = serialize(m, v))) return rc;
+  FILE *fh = fopen(path, "wb");
+  if (!fh) return fail(TM_E_ARG, "persist: cannot open %s", path);
+  size_t w = fwrite(v.data(), 1, v.size(), fh);
+  int cr = fclose(fh);
+  if (w != v.size() || cr != 0) return fail(TM_E_ARG, "persist: short write to %s", path);
+  return TM_OK;
+}
+
+// capi.c:1675-1737 and :1751-1772.
+int tm_load(const uint8_t *data, size_t len, tm_machine **out)
+{
+  if (!out) return fail(TM_E_ARG, "load: out is NULL");
+  *out = nullptr;
+  if (!data) return fail(TM_E_ARG, "load: data is NULL");
+  reader r{ data, len };
+  int32_t type;
+  uint32_t h[13];
+  double spec;
+  uint8_t tail;
+  if (!r.take(&type, 4)) return fail(TM_E_FORMAT, "load: truncated data");
+  if (type != TM_CLASSIFIER && type != TM_ENCODER) return fail(TM_E_FORMAT, "unexpected tsetlin machine type in load");
+  if (!r.take(h, sizeof h) || !r.take(&spec, 8) || !r.take(&tail, 1)) return fail(TM_E_FORMAT, "load: truncated header");
+  tm_machine *m = new tm_machine();
+  m->type = type;
+  m->classes = h[0]; m->features = h[1]; m->clauses = h[2]; m->clause_tolerance = h[3];
+  m->clause_maximum = h[4]; m->target = h[5]; m->state_bits = h[6]; m->include_bits = h[7] ? h[7] : 1;
+  m->input_bits = h[8]; m->input_chunks = h[9]; m->clause_chunks = h[10]; m->state_chunks = h[11];
+  m->action_chunks = h[12];
+  m->specificity = spec;
+  m->tail_mask = tail;
+  m->class_chunks = (m->classes + 7) / 8;
+  m->negative = m->classes ? 1.0 / (double) m->classes : 0;   // not persisted (capi.c:1509-1531)
+  if (!m->classes || !m->clauses || m->clauses % 8 || m->clause_chunks != m->clauses / 8 ||
+      m->input_bits == 0 || m->input_chunks != (m->input_bits + 7) / 8 ||
+      (uint64_t) m->action_chunks != (uint64_t) m->classes * m->clauses * m->input_chunks) {
+    delete m;
+    return fail(TM_E_FORMAT, "load: inconsistent header");
+  }
+  int rc;
+  if ((rc = size_check(m->classes, m->clauses, m->input_chunks, m->state_bits < 2 ? 2 : m->state_bits))) { delete m; return rc; }
+  m->features = m->input_bits / 2;
+  std::vector<uint8_t> acts(m->action_chunks);
+  uint8_t ind = 0;
+  if (!r.take(acts.data(), acts.size()) || !r.take(&ind, 1)) { delete m; return fail(TM_E_FORMAT, "load: truncated actions"); }
+  if (ind) { delete m; return fail(TM_E_UNSUPPORTED, "load: individualized machines are not built in this engine"); }
+  m->wd = (m->input_chunks + 3) / 4;
+  if ((rc = init_runtime(m))) { tm_destroy(m); return rc; }
+  if ((rc = m->actions.ensure(m->nclauses() * m->wd * 4))) { tm_destroy(m); return rc; }
+  if ((rc = actions_from_host(m, acts.data()))) { tm_destroy(m); return rc; }
+  m->has_state = false;   // capi.c:1696: state = NULL after load
+  m->trained = true;
+  *out = m;
+  return TM_OK;
+}
+
+int tm_load_file(const char *path, tm_machine **out)
+{
+  if (!path) return fail(TM_E_ARG, "load: path is NULL");
+  FILE *fh = fopen(path, "rb");
+  if (!fh) return fail(TM_E_ARG, "load: cannot open %s", path);
+  std::vector<uint8_t> v;
+  uint8_t tmp[65536];
+  size_t k;
+  while ((k = fread(tmp, 1, sizeof tmp, fh)) > 0) v.insert(v.end(), tmp, tmp + k);
+  fclose(fh);
+  return tm_load(v.data(), v.size(), out);
+}
+
+int tm_checkpoint(tm_machine *m, uint8_t *buf, size_t cap, size_t *len)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (!len) return fail(TM_E_ARG, "checkpoint: len is NULL");
+  *len = m->action_chunks;
+  if (!buf || cap < m->action_chunks) return fail(TM_E_SIZE, "checkpoint: buffer of %zu bytes, need %u", cap, m->action_chunks);
+  if ((rc = use_device(m))) return rc;
+  return actions_to_host(m, buf);
+}
+
+int tm_restore(tm_machine *m, const uint8_t *buf, size_t len)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (len != m->action_chunks)
+    return fail(TM_E_SIZE, "checkpoint size mismatch: expected %u bytes, got %zu", m->action_chunks, len);
+  if (!buf) return fail(TM_E_ARG, "restore: buffer is NULL");
+  if ((rc = use_device(m))) return rc;
+  return actions_from_host(m, buf);
+}
+
+// capi.c:1590-1673.
+int tm_reconfigure(tm_machine *m, const tm_options *o)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (!m->reusable) return fail(TM_E_STATE, "reconfigure requires reusable=true at creation time");
+  if (!o) return fail(TM_E_ARG, "reconfigure: options are NULL");
+  if (!o->clauses) return fail(TM_E_ARG, "reconfigure: clauses must be greater than 0");
+  if (!o->clause_tolerance) return fail(TM_E_ARG, "reconfigure: clause_tolerance must be greater than 0");
+  if (!o->clause_maximum) return fail(TM_E_ARG, "reconfigure: clause_maximum must be greater than 0");
+  if (!(o->specificity > 0)) return fail(TM_E_ARG, "reconfigure: specificity must be a positive number");
+  if ((rc = use_device(m))) return rc;
+  const uint32_t nclauses = ((o->clauses + 7) / 8) * 8;
+  if ((rc = size_check(m->classes, nclauses, m->input_chunks, m->state_bits))) return rc;
+  m->clauses = nclauses;
+  m->clause_chunks = nclauses / 8;
+  m->clause_tolerance = o->clause_tolerance;
+  m->clause_maximum = o->clause_maximum;
+  m->include_bits = o->include_bits ? o->include_bits : 1;
+  m->action_chunks = m->classes * nclauses * m->input_chunks;
+  m->state_chunks = m->classes * nclauses * (m->state_bits - 1) * m->input_chunks;
+  m->specificity = o->specificity;
+  m->target = derive_target(o->target, nclauses, o->clause_tolerance);
+  m->seed = o->seed;
+  if ((rc = alloc_device(m))) return rc;   // actions zeroed, counts (re)allocated
+  m->predict_ready = false;
+  m->trained = false;
+  return TM_OK;
+}
+
+int tm_type(const tm_machine *m)
+{
+  if (!m) return fail(TM_E_ARG, "machine is NULL");
+  return m->type;
+}
+
+void tm_destroy(tm_machine *m)
+{
+  if (!m) return;
+  if (m->stream) hipStreamSynchronize(m->stream);
+  dbuf *bufs[] = { &m->counts, &m->actions, &m->actT, &m->lit, &m->t_rows, &m->t_words, &m->t_labels,
+                   &m->t_codes, &m->t_perm, &m->t_perm_scratch, &m->t_ekeys, &m->p_rows, &m->p_words,
+                   &m->p_sums, &m->p_out, &m->t_stats };
+  for (dbuf *b : bufs) b->release();
+  if (m->ev0) hipEventDestroy(m->ev0);
+  if (m->ev1) hipEventDestroy(m->ev1);
+  if (m->stream) hipStreamDestroy(m->stream);
+  delete m;
+}
+
+int tm_get_info(const tm_machine *m, tm_info *o)
+{
+  if (!m || !o) return fail(TM_E_ARG, "get_info: NULL argument");
+  o->type = (uint32_t) m->type;
+  o->classes = m->classes; o->features = m->features; o->clauses = m->clauses;
+  o->clause_tolerance = m->clause_tolerance; o->clause_maximum = m->clause_maximum; o->target = m->target;
+  o->state_bits = m->state_bits; o->include_bits = m->include_bits; o->input_bits = m->input_bits;
+  o->input_chunks = m->input_chunks; o->clause_chunks = m->clause_chunks; o->state_chunks = m->state_chunks;
+  o->action_chunks = m->action_chunks; o->class_chunks = m->class_chunks;
+  o->specificity = m->specificity; o->negative = m->negative; o->tail_mask = m->tail_mask;
+  o->reusable = m->reusable; o->has_state = m->has_state; o->trained = m->trained;
+  return TM_OK;
+}
+
+int tm_last_train_stats(const tm_machine *m, uint64_t out[TM_NSTATS])
+{
+  if (!m || !out) return fail(TM_E_ARG, "last_train_stats: NULL argument");
+  memcpy(out, m->stats, sizeof m->stats);
+  return TM_OK;
+}
+
+int tm_last_timing(const tm_machine *m, double *train_ms, double *predict_ms)
+{
+  if (!m) return fail(TM_E_ARG, "machine is NULL");
+  if (train_ms) *train_ms = m->train_ms;
+  if (predict_ms) *predict_ms = m->predict_ms;
+  return TM_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,58 @@
+// tm_internal.h -- launch interface between the host library (tm_api.cpp) and
+// the gfx950 kernels (tm_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Training launch.  Device layouts (u32 words, Wd = ceil(input_chunks/4)):
+//   counts  [clause][plane 0..m-1][Wd]   bit-sliced counters, plane 0 = LSB
+//   actions [clause][Wd]                 the include plane (MSB of each counter)
+//   rows    [n][Wd]                      flip-interleaved literal rows
+//   perm    [epochs][n]                  sample order of each epoch
+//   ekeys   [epochs][4]                  per-epoch key words
+struct tm_train_launch {
+  uint32_t *counts;
+  uint32_t *actions;
+  const uint32_t *rows;
+  const int64_t *labels;   // classifier targets (NULL for the encoder)
+  const uint8_t *codes;    // encoder targets [n][class_chunks] (NULL for the classifier)
+  const uint32_t *perm;
+  const uint32_t *ekeys;
+  uint32_t n, wd;
+  uint32_t epoch_first, epochs;   // absolute index of the first epoch, count in this launch
+  uint32_t clause_chunks, class_chunks, total_chunks;
+  uint32_t tolerance, maximum, target, input_bits, s_draws, include_bits, state_bits;
+  uint64_t skip_threshold;        // negative sampling: skip iff draw < threshold
+  uint32_t seed_lo, seed_hi;
+  uint32_t encoder;
+  uint32_t last_live, last_eval;  // masks of the last real word (Wd-1)
+  uint32_t init;                  // 1: start from the setup state (capi.c:1136-1146)
+  unsigned long long *stats;      // optional work counters, TM_NSTATS entries (atomically added)
+};
+
+// Work counters of a train launch: TM_NSTATS entries, as documented at
+// tm_last_train_stats in tsetlin_amd.h.
+
+struct tm_predict_launch {
+  const uint32_t *actT;    // [Wd][clauses_total] transposed include plane
+  const uint32_t *lit;     // [clauses_total] literal counts
+  const uint32_t *rows;    // [n][Wd]
+  int32_t *sums;           // [n][classes]
+  uint32_t n, wd, clauses_total, clauses, classes, tolerance, last_eval;
+};
+
+// Returns a hipError_t.  `ms` receives the time of the train kernels only.
+int tm_launch_train(const tm_train_launch &L, hipStream_t st);
+int tm_launch_setup(uint32_t *counts, uint32_t *actions, uint64_t nclauses, uint32_t m, uint32_t wd, hipStream_t st);
+int tm_launch_perm(uint64_t seed, uint32_t epoch_first, uint32_t epochs, uint32_t n, uint32_t *perm_out,
+                   void *scratch, size_t scratch_bytes, hipStream_t st);
+size_t tm_perm_scratch_bytes(uint32_t n);
+int tm_launch_prepare_predict(const uint32_t *actions, uint32_t *actT, uint32_t *lit, uint32_t clauses_total,
+                              uint32_t wd, uint32_t last_eval, hipStream_t st);
+int tm_launch_predict(const tm_predict_launch &L, hipStream_t st);
+int tm_launch_argmax(const int32_t *sums, uint32_t n, uint32_t classes, int64_t *out, hipStream_t st);
+int tm_launch_encode(const int32_t *sums, uint32_t n, uint32_t classes, uint8_t *out, hipStream_t st);
+int tm_launch_repack(const uint8_t *src, uint32_t n, uint32_t bytes, uint32_t wd, uint32_t *dst, hipStream_t st);
+int tm_launch_zero(void *p, size_t bytes, hipStream_t st);
+// Train kernel tile for a row of wd words: lanes per clause and words per lane.
+int tm_train_shape(uint32_t wd, uint32_t *lanes, uint32_t *words_per_lane);

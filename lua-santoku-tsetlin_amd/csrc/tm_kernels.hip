@@ -1,0 +1,583 @@
+// tm_kernels.hip -- gfx950 kernels of the Tsetlin-machine engine.
+//
+// Train: one workgroup per 8-clause chunk (the unit of independence of the
+// reference, capi.c:201-243: feedback reads only the chunk's own 8 votes).
+// The chunk's bit-sliced automata stay in VGPRs for the whole launch; lane g of
+// clause group j owns words [g*WPL, g*WPL+WPL) of clause j's literal row in
+// every plane.  Per sample: AND-NOT + popcount per word, a group reduction of
+// the failed-literal count, an LDS exchange of the 8 signed votes, then the
+// Type I / Type II feedback on the owning lanes' registers.
+//
+// Predict: a tiled boolean contraction over (samples x clauses x words) with
+// the include plane transposed to [word][clause] so lanes (= clauses) read
+// coalesced, and the complemented literal rows staged in LDS and read as
+// wave-wide broadcasts.
+#include "tm_internal.h"
+#include "tm_rng.h"
+#include <hipcub/hipcub.hpp>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// helpers
+
+template <int G>
+__device__ __forceinline__ uint32_t group_sum(uint32_t v)
+{
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+template <int G>
+__device__ __forceinline__ bool group_any(bool pred)
+{
+  uint64_t b = __ballot(pred);
+  if (G == 64) return b != 0;
+  uint32_t lane = threadIdx.x & 63;
+  uint32_t base = lane & ~(uint32_t)(G - 1);
+  uint64_t gm = (G == 64) ? ~0ull : ((1ull << G) - 1);
+  return ((b >> base) & gm) != 0;
+}
+
+__device__ __forceinline__ uint32_t word_mask(uint32_t w, uint32_t wd, uint32_t last)
+{
+  return w + 1 < wd ? 0xFFFFFFFFu : (w + 1 == wd ? last : 0u);
+}
+
+// Saturating +1 on the literals set in c, ripple over planes 0..M (plane M is
+// the include plane); overflow ORs every plane back to one (automata.h:88-116,
+// :164-194).
+template <int M, int WPL>
+__device__ __forceinline__ void add_one(uint32_t (&st)[M + 1][WPL], int k, uint32_t c)
+{
+#pragma unroll
+  for (int p = 0; p <= M; p++) {
+    uint32_t t = st[p][k] & c;
+    st[p][k] ^= c;
+    c = t;
+  }
+#pragma unroll
+  for (int p = 0; p <= M; p++) st[p][k] |= c;
+}
+
+// Saturating -1 (automata.h:118-146, :556-594).
+template <int M, int WPL>
+__device__ __forceinline__ void sub_one(uint32_t (&st)[M + 1][WPL], int k, uint32_t b)
+{
+#pragma unroll
+  for (int p = 0; p <= M; p++) {
+    uint32_t t = ~st[p][k] & b;
+    st[p][k] ^= b;
+    b = t;
+  }
+#pragma unroll
+  for (int p = 0; p <= M; p++) st[p][k] &= ~b;
+}
+
+// include plane AND the top include_bits-1 count planes (automata.h:499-512).
+template <int M, int WPL>
+__device__ __forceinline__ uint32_t included(const uint32_t (&st)[M + 1][WPL], int k, uint32_t ib)
+{
+  uint32_t v = st[M][k];
+#pragma unroll
+  for (int i = 1; i <= M; i++)
+    if ((uint32_t) i < ib) v &= st[M - i][k];
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// training
+
+template <int WPL, int G, int M>
+__global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
+{
+  constexpr int TW = G * WPL;                  // words per clause tile
+  const uint32_t tid = threadIdx.x;
+  const uint32_t j = tid / G;                  // clause within the chunk
+  const uint32_t g = tid % G;                  // lane within the clause group
+  const uint32_t lane = tid & 63;
+  const uint32_t ch = blockIdx.x;
+  const uint32_t cls = ch / L.clause_chunks;
+  const uint32_t clause = ch * 8 + j;
+  const uint32_t wbase = g * WPL;
+  const uint32_t wd = L.wd;
+
+  __shared__ int32_t votes_lds[2][8];
+  __shared__ uint32_t hits[8][TW];
+
+  for (uint32_t i = tid; i < 8 * TW; i += 8 * G) (&hits[0][0])[i] = 0;
+
+  uint32_t st[M + 1][WPL];
+#pragma unroll
+  for (int k = 0; k < WPL; k++) {
+    uint32_t w = wbase + k;
+    if (L.init) {
+#pragma unroll
+      for (int p = 0; p < M; p++) st[p][k] = 0xFFFFFFFFu;
+      st[M][k] = 0;
+    } else if (w < wd) {
+#pragma unroll
+      for (int p = 0; p < M; p++) st[p][k] = L.counts[((size_t) clause * M + p) * wd + w];
+      st[M][k] = L.actions[(size_t) clause * wd + w];
+    } else {
+#pragma unroll
+      for (int p = 0; p <= M; p++) st[p][k] = 0;
+    }
+  }
+
+  auto recount = [&]() -> uint32_t {
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < WPL; k++) c += __popc(st[M][k] & word_mask(wbase + k, wd, L.last_eval));
+    return group_sum<G>(c);
+  };
+  uint32_t lit = recount();
+  __syncthreads();
+
+  const uint32_t T = L.target;
+  const uint32_t tol = L.tolerance;
+  uint32_t buf = 0;
+  uint32_t n_steps = 0, n_t1 = 0, n_grow = 0, n_t2 = 0, n_t1z = 0, n_draws = 0;
+
+  for (uint32_t e = 0; e < L.epochs; e++) {
+    const uint32_t epoch = L.epoch_first + e;
+    const uint32_t *perm = L.perm + (size_t) e * L.n;
+    const uint32_t ek0 = L.ekeys[4 * e + 0], ek1 = L.ekeys[4 * e + 1];
+    const uint32_t ek2 = L.ekeys[4 * e + 2], ek3 = L.ekeys[4 * e + 3];
+    for (uint32_t b0 = 0; b0 < L.n; b0 += 64) {
+      const uint32_t pos = b0 + lane;
+      const bool valid = pos < L.n;
+      const uint32_t s = valid ? perm[pos] : 0u;
+      bool tgt, want;
+      if (L.encoder) {
+        tgt = valid && ((L.codes[(size_t) s * L.class_chunks + (cls >> 3)] >> (cls & 7)) & 1);
+        want = valid;
+      } else {
+        const uint32_t sc = valid ? (uint32_t) L.labels[s] : 0xFFFFFFFFu;
+        tgt = valid && sc == cls;
+        want = valid && (tgt || (uint64_t) tm_skip_draw(ek0, ek1, s, ch) >= L.skip_threshold);
+      }
+      uint64_t mask = __ballot(want);
+      if (!mask) continue;
+      const uint64_t tmask = __ballot(tgt);
+      const tm_u4 d = tm_philox(s, ch, epoch, TM_TAG_STEP, L.seed_lo, L.seed_hi);
+      while (mask) {
+        const int q = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const uint32_t sq = __builtin_amdgcn_readlane(s, q);
+        const bool tq = (tmask >> q) & 1;
+        const uint32_t d0 = __builtin_amdgcn_readlane(d.x, q), d1 = __builtin_amdgcn_readlane(d.y, q);
+        const uint32_t d2 = __builtin_amdgcn_readlane(d.z, q), d3 = __builtin_amdgcn_readlane(d.w, q);
+        const uint32_t dsel = (j >> 1) == 0 ? d0 : (j >> 1) == 1 ? d1 : (j >> 1) == 2 ? d2 : d3;
+        const uint32_t u16 = (dsel >> (16 * (j & 1))) & 0xFFFFu;
+
+        uint32_t x[WPL];
+        const uint32_t *row = L.rows + (size_t) sq * wd;
+#pragma unroll
+        for (int k = 0; k < WPL; k++) x[k] = (wbase + k < wd) ? row[wbase + k] : 0u;
+
+        uint32_t f = 0;
+#pragma unroll
+        for (int k = 0; k < WPL; k++) f += __popc(st[M][k] & ~x[k] & word_mask(wbase + k, wd, L.last_eval));
+        f = group_sum<G>(f);
+        int32_t votes;
+        if (lit == 0) votes = (int32_t) tol;
+        else {
+          votes = (int32_t)(lit < tol ? lit : tol) - (int32_t) f;
+          if (votes < 0) votes = 0;
+        }
+        if (g == 0) votes_lds[buf][j] = (j & 1) ? -votes : votes;
+        __syncthreads();
+        int32_t v = 0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) v += votes_lds[buf][c];
+        buf ^= 1;
+        if (v > (int32_t) T) v = (int32_t) T;
+        if (v < -(int32_t) T) v = -(int32_t) T;
+        const uint64_t num = (uint64_t)(tq ? (int64_t) T - v : (int64_t) T + v);
+        const bool fire = (uint64_t) u16 * (2ull * T) < (num << 16);
+        n_steps++;
+        if (!fire) continue;
+        const bool type1 = tq ? !(j & 1) : (j & 1);
+        const bool out = votes > 0;
+        if (type1 && out) {
+          // Type I, clause true: inc true literals (if short enough), then
+          // dec false literals that are not included (capi.c:179-183).
+          const bool grow = lit <= L.maximum;
+          n_t1++;
+          n_grow += grow;
+#pragma unroll
+          for (int k = 0; k < WPL; k++) {
+            const uint32_t live = word_mask(wbase + k, wd, L.last_live);
+            if (grow) add_one<M, WPL>(st, k, x[k] & live);
+            sub_one<M, WPL>(st, k, ~x[k] & ~included<M, WPL>(st, k, L.include_bits) & live);
+          }
+          lit = recount();
+        } else if (type1) {
+          // Type I, clause false: s_draws random single-literal decrements
+          // (capi.c:184-192).  Saturating decrements commute, so each round
+          // applies one decrement to every distinct literal hit by the draws
+          // still pending; a draw whose literal was already taken this round
+          // waits for the next one.
+          const uint32_t kev = tm_literal_key(ek2, ek3, sq, clause);
+          n_t1z++;
+          n_draws += L.s_draws;
+          const uint32_t nd = (L.s_draws + G - 1) / G;
+          for (uint32_t base = 0; base < nd; base += 32) {
+            const uint32_t cnt = min(32u, nd - base);
+            uint32_t pend = 0;
+            for (uint32_t i = 0; i < cnt; i++)
+              if ((base + i) * G + g < L.s_draws) pend |= 1u << i;
+            while (group_any<G>(pend != 0)) {
+              uint32_t pp = pend;
+              while (pp) {
+                const uint32_t i = __builtin_ctz(pp);
+                pp &= pp - 1;
+                const uint32_t lbit = tm_literal_draw(kev, (base + i) * G + g, L.input_bits);
+                const uint32_t bm = 1u << (lbit & 31);
+                const uint32_t old = atomicOr(&hits[j][lbit >> 5], bm);
+                if (!(old & bm)) pend &= ~(1u << i);
+              }
+              __threadfence_block();
+#pragma unroll
+              for (int k = 0; k < WPL; k++) {
+                const uint32_t hm = hits[j][wbase + k];
+                hits[j][wbase + k] = 0;
+                sub_one<M, WPL>(st, k, hm);
+              }
+              __threadfence_block();
+            }
+          }
+          lit = recount();
+        } else if (out) {
+          // Type II, clause true: inc false literals that are not included
+          // (capi.c:194-197).
+          n_t2++;
+#pragma unroll
+          for (int k = 0; k < WPL; k++) {
+            const uint32_t live = word_mask(wbase + k, wd, L.last_live);
+            add_one<M, WPL>(st, k, ~x[k] & ~included<M, WPL>(st, k, L.include_bits) & live);
+          }
+          lit = recount();
+        }
+      }
+    }
+  }
+
+  if (L.stats && g == 0) {
+    if (j == 0) atomicAdd(&L.stats[0], (unsigned long long) n_steps);
+    atomicAdd(&L.stats[1], (unsigned long long) n_t1);
+    atomicAdd(&L.stats[2], (unsigned long long) n_grow);
+    atomicAdd(&L.stats[3], (unsigned long long) n_t2);
+    atomicAdd(&L.stats[4], (unsigned long long) n_t1z);
+    atomicAdd(&L.stats[5], (unsigned long long) n_draws);
+  }
+
+#pragma unroll
+  for (int k = 0; k < WPL; k++) {
+    const uint32_t w = wbase + k;
+    if (w < wd) {
+#pragma unroll
+      for (int p = 0; p < M; p++) L.counts[((size_t) clause * M + p) * wd + w] = st[p][k];
+      L.actions[(size_t) clause * wd + w] = st[M][k];
+    }
+  }
+}
+
+__global__ void setup_kernel(uint32_t *counts, uint32_t *actions, uint64_t ncount_words, uint64_t naction_words)
+{
+  uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t) gridDim.x * blockDim.x;
+  for (uint64_t t = i; t < ncount_words; t += stride) counts[t] = 0xFFFFFFFFu;
+  for (uint64_t t = i; t < naction_words; t += stride) actions[t] = 0;
+}
+
+__global__ void perm_keys_kernel(uint32_t seed_lo, uint32_t seed_hi, uint32_t epoch, uint32_t n, uint32_t *keys, uint32_t *vals)
+{
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = tm_philox(i, epoch, 0, TM_TAG_PERM, seed_lo, seed_hi).x;
+  vals[i] = i;
+}
+
+// ---------------------------------------------------------------------------
+// predict
+
+// actT[w][c] = actions[c][w]; 32x32 tiles through LDS.
+__global__ void transpose_kernel(const uint32_t *actions, uint32_t *actT, uint32_t nclauses, uint32_t wd)
+{
+  __shared__ uint32_t tile[32][33];
+  const uint32_t c0 = blockIdx.x * 32, w0 = blockIdx.y * 32;
+  const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 256 threads: 32 x 8
+  for (uint32_t r = ty; r < 32; r += 8) {
+    uint32_t c = c0 + r, w = w0 + tx;
+    tile[r][tx] = (c < nclauses && w < wd) ? actions[(size_t) c * wd + w] : 0u;
+  }
+  __syncthreads();
+  for (uint32_t r = ty; r < 32; r += 8) {
+    uint32_t w = w0 + r, c = c0 + tx;
+    if (c < nclauses && w < wd) actT[(size_t) w * nclauses + c] = tile[tx][r];
+  }
+}
+
+__global__ void literal_count_kernel(const uint32_t *actions, uint32_t *lit, uint32_t nclauses, uint32_t wd, uint32_t last_eval)
+{
+  const uint32_t c = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const uint32_t lane = threadIdx.x & 63;
+  if (c >= nclauses) return;
+  uint32_t n = 0;
+  for (uint32_t w = lane; w < wd; w += 64) n += __popc(actions[(size_t) c * wd + w] & word_mask(w, wd, last_eval));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+  if (lane == 0) lit[c] = n;
+}
+
+constexpr int PRED_WAVES = 4;    // waves per workgroup
+constexpr int PRED_SPW = 16;     // samples per wave
+constexpr int PRED_WC = 32;      // words per staged slice
+
+__global__ __launch_bounds__(64 * PRED_WAVES) void predict_kernel(tm_predict_launch L)
+{
+  __shared__ uint4 xs[PRED_WAVES * PRED_SPW][PRED_WC / 4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t c = blockIdx.y * 64 + lane;
+  const uint32_t s0 = blockIdx.x * (PRED_WAVES * PRED_SPW);
+  const bool cvalid = c < L.clauses_total;
+  uint32_t acc[PRED_SPW];
+#pragma unroll
+  for (int i = 0; i < PRED_SPW; i++) acc[i] = 0;
+
+  for (uint32_t w0 = 0; w0 < L.wd; w0 += PRED_WC) {
+    __syncthreads();
+    uint32_t *xw = reinterpret_cast<uint32_t *>(&xs[0][0]);
+    for (uint32_t i = tid; i < PRED_WAVES * PRED_SPW * PRED_WC; i += 64 * PRED_WAVES) {
+      const uint32_t sl = i / PRED_WC, ww = i % PRED_WC, s = s0 + sl, w = w0 + ww;
+      uint32_t v = 0;
+      if (s < L.n && w < L.wd) v = ~L.rows[(size_t) s * L.wd + w] & word_mask(w, L.wd, L.last_eval);
+      xw[i] = v;
+    }
+    __syncthreads();
+    uint32_t a[PRED_WC];
+#pragma unroll
+    for (int ww = 0; ww < PRED_WC; ww++) {
+      const uint32_t w = w0 + ww;
+      a[ww] = (cvalid && w < L.wd) ? L.actT[(size_t) w * L.clauses_total + c] : 0u;
+    }
+#pragma unroll
+    for (int si = 0; si < PRED_SPW; si++) {
+      const uint32_t sl = wave * PRED_SPW + si;
+      uint32_t t = acc[si];
+#pragma unroll
+      for (int q = 0; q < PRED_WC / 4; q++) {
+        const uint4 xv = xs[sl][q];
+        t += __popc(a[4 * q + 0] & xv.x);
+        t += __popc(a[4 * q + 1] & xv.y);
+        t += __popc(a[4 * q + 2] & xv.z);
+        t += __popc(a[4 * q + 3] & xv.w);
+      }
+      acc[si] = t;
+    }
+  }
+
+  // votes (capi.c:110-119) and class sums (capi.c:928-933)
+  const uint32_t lit = cvalid ? L.lit[c] : 0u;
+  const uint32_t cbase = blockIdx.y * 64;
+  const uint32_t cls_first = cbase / L.clauses;
+  uint32_t clast = cbase + 63;
+  if (clast >= L.clauses_total) clast = L.clauses_total - 1;
+  const uint32_t cls_last = clast / L.clauses;
+  const uint32_t my_cls = c / L.clauses;
+#pragma unroll
+  for (int si = 0; si < PRED_SPW; si++) {
+    const uint32_t s = s0 + wave * PRED_SPW + si;
+    int32_t v = 0;
+    if (cvalid) {
+      if (lit == 0) v = (int32_t) L.tolerance;
+      else {
+        v = (int32_t)(lit < L.tolerance ? lit : L.tolerance) - (int32_t) acc[si];
+        if (v < 0) v = 0;
+      }
+      if (c & 1) v = -v;
+    }
+    for (uint32_t k = cls_first; k <= cls_last; k++) {
+      int32_t t = (my_cls == k) ? v : 0;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+      if (lane == 0 && s < L.n) atomicAdd(&L.sums[(size_t) s * L.classes + k], t);
+    }
+  }
+}
+
+// argmax with strict '>' from -inf: ties go to the lowest class (capi.c:934-942).
+__global__ void argmax_kernel(const int32_t *sums, uint32_t n, uint32_t classes, int64_t *out)
+{
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int32_t *r = sums + (size_t) s * classes;
+  int64_t best = -INT64_MAX;
+  uint32_t bc = 0;
+  for (uint32_t k = 0; k < classes; k++)
+    if ((int64_t) r[k] > best) { best = r[k]; bc = k; }
+  out[s] = bc;
+}
+
+// bit h = (sum_h > 0), LSB first; padding bits zero (capi.c:984-990).
+__global__ void encode_kernel(const int32_t *sums, uint32_t n, uint32_t classes, uint8_t *out)
+{
+  const uint32_t cc = (classes + 7) / 8;
+  uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t) n * cc) return;
+  const uint32_t s = (uint32_t)(i / cc), b = (uint32_t)(i % cc);
+  uint8_t v = 0;
+  for (uint32_t t = 0; t < 8; t++) {
+    uint32_t k = b * 8 + t;
+    if (k < classes && sums[(size_t) s * classes + k] > 0) v |= (uint8_t)(1u << t);
+  }
+  out[i] = v;
+}
+
+__global__ void repack_kernel(const uint8_t *src, uint32_t n, uint32_t bytes, uint32_t wd, uint32_t *dst)
+{
+  uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t) n * wd) return;
+  const uint64_t s = i / wd, w = i % wd;
+  uint32_t v = 0;
+  for (uint32_t b = 0; b < 4; b++) {
+    uint64_t o = w * 4 + b;
+    if (o < bytes) v |= (uint32_t) src[s * bytes + o] << (8 * b);
+  }
+  dst[i] = v;
+}
+
+__global__ void zero_kernel(uint32_t *p, uint64_t nwords)
+{
+  uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t) gridDim.x * blockDim.x;
+  for (; i < nwords; i += stride) p[i] = 0;
+}
+
+template <int WPL, int G>
+int launch_train_shape(const tm_train_launch &L, hipStream_t st)
+{
+  if (L.state_bits != 8) return (int) hipErrorInvalidValue;
+  hipLaunchKernelGGL((train_kernel<WPL, G, 7>), dim3(L.total_chunks), dim3(8 * G), 0, st, L);
+  return (int) hipGetLastError();
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+
+int tm_train_shape(uint32_t wd, uint32_t *lanes, uint32_t *wpl)
+{
+  static const uint32_t shapes[][2] = { { 8, 1 }, { 16, 1 }, { 32, 1 }, { 64, 1 }, { 64, 2 }, { 64, 4 }, { 64, 8 }, { 64, 16 } };
+  for (auto &s : shapes)
+    if (s[0] * s[1] >= wd) { *lanes = s[0]; *wpl = s[1]; return 0; }
+  return -1;
+}
+
+int tm_launch_train(const tm_train_launch &L, hipStream_t st)
+{
+  uint32_t G, W;
+  if (tm_train_shape(L.wd, &G, &W)) return (int) hipErrorInvalidValue;
+  switch (G * 100 + W) {
+    case 801: return launch_train_shape<1, 8>(L, st);
+    case 1601: return launch_train_shape<1, 16>(L, st);
+    case 3201: return launch_train_shape<1, 32>(L, st);
+    case 6401: return launch_train_shape<1, 64>(L, st);
+    case 6402: return launch_train_shape<2, 64>(L, st);
+    case 6404: return launch_train_shape<4, 64>(L, st);
+    case 6408: return launch_train_shape<8, 64>(L, st);
+    case 6416: return launch_train_shape<16, 64>(L, st);
+  }
+  return (int) hipErrorInvalidValue;
+}
+
+int tm_launch_setup(uint32_t *counts, uint32_t *actions, uint64_t nclauses, uint32_t m, uint32_t wd, hipStream_t st)
+{
+  hipLaunchKernelGGL(setup_kernel, dim3(1024), dim3(256), 0, st, counts, actions,
+                     nclauses * m * wd, nclauses * wd);
+  return (int) hipGetLastError();
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t) 255; }
+
+static size_t perm_sort_temp_bytes(uint32_t n)
+{
+  size_t temp = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (uint32_t *) nullptr, (uint32_t *) nullptr,
+                                     (uint32_t *) nullptr, (uint32_t *) nullptr, (int) n);
+  return align256(temp);
+}
+
+// scratch layout: [sort temp storage][keys n][sorted keys n][values n], each 256-aligned
+size_t tm_perm_scratch_bytes(uint32_t n)
+{
+  return perm_sort_temp_bytes(n) + 3 * align256((size_t) n * sizeof(uint32_t));
+}
+
+int tm_launch_perm(uint64_t seed, uint32_t epoch_first, uint32_t epochs, uint32_t n, uint32_t *perm_out,
+                   void *scratch, size_t scratch_bytes, hipStream_t st)
+{
+  const size_t temp_bytes = perm_sort_temp_bytes(n);
+  const size_t arr = align256((size_t) n * sizeof(uint32_t));
+  if (scratch_bytes < temp_bytes + 3 * arr) return (int) hipErrorInvalidValue;
+  char *temp = (char *) scratch;
+  uint32_t *keys = (uint32_t *)(temp + temp_bytes);
+  uint32_t *keys_sorted = (uint32_t *)(temp + temp_bytes + arr);
+  uint32_t *vals = (uint32_t *)(temp + temp_bytes + 2 * arr);
+  for (uint32_t e = 0; e < epochs; e++) {
+    hipLaunchKernelGGL(perm_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, st,
+                       (uint32_t) seed, (uint32_t)(seed >> 32), epoch_first + e, n, keys, vals);
+    size_t tb = temp_bytes;
+    hipError_t err = hipcub::DeviceRadixSort::SortPairs(temp, tb, keys, keys_sorted, vals,
+                                                        perm_out + (size_t) e * n, (int) n, 0, 32, st);
+    if (err != hipSuccess) return (int) err;
+  }
+  return (int) hipGetLastError();
+}
+
+int tm_launch_prepare_predict(const uint32_t *actions, uint32_t *actT, uint32_t *lit, uint32_t nclauses,
+                              uint32_t wd, uint32_t last_eval, hipStream_t st)
+{
+  hipLaunchKernelGGL(transpose_kernel, dim3((nclauses + 31) / 32, (wd + 31) / 32), dim3(256), 0, st,
+                     actions, actT, nclauses, wd);
+  hipLaunchKernelGGL(literal_count_kernel, dim3((nclauses + 3) / 4), dim3(256), 0, st,
+                     actions, lit, nclauses, wd, last_eval);
+  return (int) hipGetLastError();
+}
+
+int tm_launch_predict(const tm_predict_launch &L, hipStream_t st)
+{
+  dim3 grid((L.n + PRED_WAVES * PRED_SPW - 1) / (PRED_WAVES * PRED_SPW), (L.clauses_total + 63) / 64);
+  hipLaunchKernelGGL(predict_kernel, grid, dim3(64 * PRED_WAVES), 0, st, L);
+  return (int) hipGetLastError();
+}
+
+int tm_launch_argmax(const int32_t *sums, uint32_t n, uint32_t classes, int64_t *out, hipStream_t st)
+{
+  hipLaunchKernelGGL(argmax_kernel, dim3((n + 255) / 256), dim3(256), 0, st, sums, n, classes, out);
+  return (int) hipGetLastError();
+}
+
+int tm_launch_encode(const int32_t *sums, uint32_t n, uint32_t classes, uint8_t *out, hipStream_t st)
+{
+  uint64_t total = (uint64_t) n * ((classes + 7) / 8);
+  hipLaunchKernelGGL(encode_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, sums, n, classes, out);
+  return (int) hipGetLastError();
+}
+
+int tm_launch_repack(const uint8_t *src, uint32_t n, uint32_t bytes, uint32_t wd, uint32_t *dst, hipStream_t st)
+{
+  uint64_t total = (uint64_t) n * wd;
+  hipLaunchKernelGGL(repack_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, src, n, bytes, wd, dst);
+  return (int) hipGetLastError();
+}
+
+int tm_launch_zero(void *p, size_t bytes, hipStream_t st)
+{
+  hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(256), 0, st, (uint32_t *) p, (uint64_t)(bytes / 4));
+  return (int) hipGetLastError();
+}

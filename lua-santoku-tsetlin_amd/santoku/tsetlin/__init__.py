@@ -1,0 +1,239 @@
+"""santoku.tsetlin -- host mirror of the reference's Lua module over the C ABI.
+
+Mirrors module santoku.tsetlin.capi (lib/santoku/tsetlin/capi.c:1790-1804,
+re-exported by lib/santoku/tsetlin.lua:1): ``create(type, opts)``,
+``load(data, from_string)``, ``align``; and the machine methods of the
+``santoku_tsetlin`` metatable (capi.c:419-431): train, predict, persist,
+checkpoint, restore, reconfigure, type, destroy.  The README spellings
+``classifier(opts)`` / ``encoder(opts)`` are exported too (README.md:17,32).
+
+Errors the reference raises with luaL_error / tk_lua_verror raise TsetlinError
+here, carrying the engine's message.  Byte buffers (problems, sentences, codes,
+checkpoints) may be bytes, bytearray, numpy uint8 arrays or, for train and
+predict, torch tensors on the GPU (taken as device pointers, no copy).
+"""
+import ctypes
+import sys
+
+import numpy as np
+
+from . import capi as _c
+
+align = 8
+
+
+class TsetlinError(RuntimeError):
+    """Raised where the reference raises a Lua error."""
+
+
+def _check(rc):
+    if rc != _c.TM_OK:
+        msg = _c.load_library().tm_last_error()
+        raise TsetlinError("%s: %s" % (_c.ERRORS.get(rc, rc), msg.decode() if msg else ""))
+
+
+def _is_device_tensor(x):
+    t = type(x)
+    return t.__module__.startswith("torch") and t.__name__ == "Tensor" and x.is_cuda
+
+
+def _host_buffer(x, dtype=np.uint8):
+    """(array, pointer) for a host buffer; keeps the array alive."""
+    if isinstance(x, (bytes, bytearray, memoryview)):
+        a = np.frombuffer(x, dtype=np.uint8)
+        if dtype != np.uint8:
+            a = a.view(dtype)
+    else:
+        a = np.ascontiguousarray(np.asarray(x), dtype=dtype)
+    return a, a.ctypes.data
+
+
+def _options(o):
+    lib = _c.load_library()
+    opts = _c.Options()
+    lib.tm_options_default(ctypes.byref(opts))
+    for key, field in (("classes", "classes"), ("hidden", "classes"), ("features", "features"),
+                       ("visible", "features"), ("clauses", "clauses"), ("clause_tolerance", "clause_tolerance"),
+                       ("clause_maximum", "clause_maximum"), ("state", "state"), ("include_bits", "include_bits"),
+                       ("target", "target"), ("negative", "negative"), ("specificity", "specificity"),
+                       ("seed", "seed")):
+        if o.get(key) is not None:
+            setattr(opts, field, type(getattr(opts, field))(o[key]))
+    opts.reusable = 1 if o.get("reusable") else 0
+    opts.individualized = 1 if o.get("individualized") else 0
+    return opts
+
+
+class Machine:
+    """A classifier or encoder (the reference's tk_tsetlin_t userdata)."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+        self._lib = _c.load_library()
+
+    def _handle(self):
+        if not self._h:
+            raise TsetlinError("machine destroyed")
+        return self._h
+
+    # capi.c:1774-1788
+    def type(self):
+        t = self._lib.tm_type(self._handle())
+        if t < 0:
+            _check(t)
+        return "classifier" if t == _c.TM_CLASSIFIER else "encoder"
+
+    def info(self):
+        i = _c.Info()
+        _check(self._lib.tm_get_info(self._handle(), ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in _c.Info._fields_}
+
+    # capi.c:1484-1507 (dispatch), :1112-1191 (classifier), :1193-1282 (encoder)
+    def train(self, opts=None, **kw):
+        o = dict(opts or {}, **kw)
+        n = int(o["samples"])
+        iters = int(o["iterations"])
+        each = o.get("each")
+        enc = self.type() == "encoder"
+        rows = o["sentences"] if enc else o["problems"]
+        tgt = o["codes"] if enc else o["solutions"]
+        keep = []
+
+        def on_epoch(_user, epoch):
+            try:
+                r = each(epoch)
+            except Exception as ex:  # the reference prints and stops (capi.c:1171-1175)
+                print("Error in Lua callback: %s" % ex, file=sys.stderr)
+                return -1
+            return 0 if r is False else 1
+
+        cb = _c.EPOCH_FN(on_epoch) if each is not None else _c.EPOCH_FN()
+        if _is_device_tensor(rows):
+            rp, tp = rows.data_ptr(), tgt.data_ptr()
+            fn = self._lib.tm_train_encoder_dev if enc else self._lib.tm_train_classifier_dev
+        else:
+            ra, rp = _host_buffer(rows)
+            ta, tp = _host_buffer(tgt, np.uint8 if enc else np.int64)
+            keep += [ra, ta]
+            fn = self._lib.tm_train_encoder if enc else self._lib.tm_train_classifier
+        _check(fn(self._handle(), n, rp, tp, iters, cb, None))
+
+    # capi.c:1088-1110, :903-948, :950-995
+    def predict(self, problems, n, threads=None):
+        n = int(n)
+        enc = self.type() == "encoder"
+        if _is_device_tensor(problems):
+            import torch
+            if enc:
+                cc = (self.info()["classes"] + 7) // 8
+                out = torch.empty((n, cc), dtype=torch.uint8, device=problems.device)
+                _check(self._lib.tm_predict_encoder_dev(self._handle(), problems.data_ptr(), n, out.data_ptr()))
+            else:
+                out = torch.empty(n, dtype=torch.int64, device=problems.device)
+                _check(self._lib.tm_predict_classifier_dev(self._handle(), problems.data_ptr(), n, out.data_ptr()))
+            return out
+        pa, pp = _host_buffer(problems)
+        if enc:
+            cc = (self.info()["classes"] + 7) // 8
+            out = np.zeros((n, cc), np.uint8)
+            _check(self._lib.tm_predict_encoder(self._handle(), pp, n, out.ctypes.data))
+        else:
+            out = np.zeros(n, np.int64)
+            _check(self._lib.tm_predict_classifier(self._handle(), pp, n, out.ctypes.data))
+        return out
+
+    # capi.c:1538-1562: returns the bytes when path is None
+    def persist(self, path=None, persist_state=None):
+        if path is not None:
+            _check(self._lib.tm_persist_file(self._handle(), str(path).encode()))
+            return None
+        ln = ctypes.c_size_t(0)
+        rc = self._lib.tm_persist(self._handle(), None, 0, ctypes.byref(ln))
+        if rc not in (_c.TM_OK, -5):
+            _check(rc)
+        buf = np.zeros(ln.value, np.uint8)
+        _check(self._lib.tm_persist(self._handle(), buf.ctypes.data, ln.value, ctypes.byref(ln)))
+        return buf.tobytes()
+
+    # capi.c:1564-1576: the buffer is resized by the callee
+    def checkpoint(self, buf=None):
+        ln = ctypes.c_size_t(0)
+        rc = self._lib.tm_checkpoint(self._handle(), None, 0, ctypes.byref(ln))
+        if rc not in (_c.TM_OK, -5):
+            _check(rc)
+        out = np.zeros(ln.value, np.uint8)
+        _check(self._lib.tm_checkpoint(self._handle(), out.ctypes.data, ln.value, ctypes.byref(ln)))
+        if isinstance(buf, bytearray):
+            buf[:] = out.tobytes()
+            return buf
+        return out
+
+    # capi.c:1578-1588
+    def restore(self, buf):
+        a, p = _host_buffer(buf)
+        _check(self._lib.tm_restore(self._handle(), p, a.nbytes))
+
+    # capi.c:1590-1673
+    def reconfigure(self, opts=None, **kw):
+        o = dict(opts or {}, **kw)
+        if "include_bits" not in o:
+            o["include_bits"] = self.info()["include_bits"]
+        _check(self._lib.tm_reconfigure(self._handle(), ctypes.byref(_options(o))))
+
+    # capi.c:743-750
+    def destroy(self):
+        if self._h:
+            self._lib.tm_destroy(self._h)
+            self._h = ctypes.c_void_p(None)
+
+    def timing(self):
+        t, p = ctypes.c_double(0), ctypes.c_double(0)
+        _check(self._lib.tm_last_timing(self._handle(), ctypes.byref(t), ctypes.byref(p)))
+        return t.value, p.value
+
+    def train_stats(self):
+        """Work counters of the last train call (see tm_last_train_stats)."""
+        out = (ctypes.c_uint64 * 6)()
+        _check(self._lib.tm_last_train_stats(self._handle(), out))
+        keys = ("steps", "type1_true", "type1_grow", "type2_true", "type1_false", "draws")
+        return dict(zip(keys, [int(v) for v in out]))
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+# capi.c:1702-1724 (create dispatch)
+def create(type_, opts=None, **kw):
+    o = dict(opts or {}, **kw)
+    lib = _c.load_library()
+    h = ctypes.c_void_p()
+    if type_ == "classifier":
+        _check(lib.tm_create_classifier(ctypes.byref(_options(o)), ctypes.byref(h)))
+    elif type_ == "encoder":
+        _check(lib.tm_create_encoder(ctypes.byref(_options(o)), ctypes.byref(h)))
+    else:
+        raise TsetlinError("unexpected tsetlin machine type in create")
+    return Machine(h.value)
+
+
+def classifier(opts=None, **kw):
+    return create("classifier", opts, **kw)
+
+
+def encoder(opts=None, **kw):
+    return create("encoder", opts, **kw)
+
+
+# capi.c:1751-1772: a path, or the persisted bytes when from_string is true
+def load(data, from_string=False, read_state=None):
+    lib = _c.load_library()
+    h = ctypes.c_void_p()
+    if from_string:
+        a, p = _host_buffer(data)
+        _check(lib.tm_load(p, a.nbytes, ctypes.byref(h)))
+    else:
+        _check(lib.tm_load_file(str(data).encode(), ctypes.byref(h)))
+    return Machine(h.value)

@@ -1,0 +1,117 @@
+"""Literal-row packing and the synthetic workloads the engine is measured on.
+
+``bits_to_cvec`` produces the engine's input contract: one flip-interleaved row
+per sample, ``ceil(2F/8)`` bytes, bits ``0..F-1`` = x and bits ``F..2F-1`` = not
+x, least-significant bit first, padding bits zero.  It follows the layout of
+santoku-matrix's ``ivec:bits_to_cvec(n, F, true)`` as the reference documents it
+(doc/sth.md:85-91) and uses it (test/spec/santoku/tsetlin/classify/mnist.lua:60).
+"""
+import numpy as np
+
+
+def bits_to_cvec(x):
+    """x: (n, F) bool/0-1 array -> (n, ceil(2F/8)) uint8 flip-interleaved rows."""
+    x = np.asarray(x).astype(bool, copy=False)
+    lits = np.concatenate([x, ~x], axis=1)
+    return np.packbits(lits, axis=1, bitorder="little")
+
+
+def codes_from_bits(h):
+    """h: (n, H) bool -> (n, ceil(H/8)) uint8 codes, LSB first (capi.c:1254)."""
+    return np.packbits(np.asarray(h).astype(bool, copy=False), axis=1, bitorder="little")
+
+
+# ---------------------------------------------------------------------------
+# Workload 1: plain classifier, CPU-runnable (8192 train + 2048 test,
+# 256 Bernoulli(0.5) features, 4 classes, 5% label flips).
+# ---------------------------------------------------------------------------
+
+def xor_and_task(seed, n_train=8192, n_test=2048, features=256, flip=0.05):
+    """class = 2*(x3 AND x17) + (x40 XOR x91); a flipped label moves to one of
+    the 3 other classes uniformly (Bayes ceiling 95%)."""
+    rng = np.random.default_rng(seed)
+    n = n_train + n_test
+    x = rng.random((n, features)) < 0.5
+    y = 2 * (x[:, 3] & x[:, 17]).astype(np.int64) + (x[:, 40] ^ x[:, 91]).astype(np.int64)
+    flip_mask = rng.random(n) < flip
+    y[flip_mask] = (y[flip_mask] + rng.integers(1, 4, flip_mask.sum())) % 4
+    rows = bits_to_cvec(x)
+    return dict(train_rows=rows[:n_train], train_y=y[:n_train], test_rows=rows[n_train:], test_y=y[n_train:],
+                features=features, classes=4, x=x)
+
+
+# ---------------------------------------------------------------------------
+# Workload 2: plain classifier at 2048 clauses/class (200k train + 20k test,
+# 4096 sparse Bernoulli(0.05) features, 20 classes from planted conjunctions).
+# ---------------------------------------------------------------------------
+
+def planted_conjunctions(seed, classes=20, features=4096, per_class=3):
+    """Three conjunctions of 2..4 literals per class 1..classes-1.  Each has one
+    positive literal and the rest negated, so it holds with probability about
+    0.05 * 0.95^(k-1) at density 0.05 (all-positive conjunctions would almost
+    never fire)."""
+    rng = np.random.default_rng(seed ^ 0x5EED)
+    conj = []
+    for c in range(1, classes):
+        for _ in range(per_class):
+            k = int(rng.integers(2, 5))
+            feats = rng.choice(features, size=k, replace=False)
+            pol = np.zeros(k, bool)
+            pol[0] = True
+            conj.append((c, feats, pol))
+    return conj
+
+
+def label_planted(x, conj, classes):
+    n = x.shape[0]
+    y = np.zeros(n, np.int64)
+    done = np.zeros(n, bool)
+    for c, feats, pol in conj:
+        m = np.ones(n, bool)
+        for f, p in zip(feats, pol):
+            m &= x[:, f] if p else ~x[:, f]
+        hit = m & ~done
+        y[hit] = c
+        done |= hit
+    return y
+
+
+def sparse_planted_task(seed, n_train=200_000, n_test=20_000, features=4096, classes=20,
+                        density=0.05, noise=0.02, block=8192):
+    """First matching conjunction wins (class order 1..classes-1), else class 0;
+    2% of labels are replaced by a uniformly random different class.  Rows are
+    generated in blocks so peak memory stays at a few hundred MB."""
+    rng = np.random.default_rng(seed)
+    conj = planted_conjunctions(seed, classes, features)
+    n = n_train + n_test
+    rows = np.empty((n, (2 * features + 7) // 8), np.uint8)
+    y = np.empty(n, np.int64)
+    for s in range(0, n, block):
+        e = min(n, s + block)
+        xb = rng.random((e - s, features), dtype=np.float32) < density
+        y[s:e] = label_planted(xb, conj, classes)
+        rows[s:e] = bits_to_cvec(xb)
+    flip = rng.random(n) < noise
+    y[flip] = (y[flip] + rng.integers(1, classes, flip.sum())) % classes
+    return dict(train_rows=rows[:n_train], train_y=y[:n_train], test_rows=rows[n_train:], test_y=y[n_train:],
+                features=features, classes=classes, conj=conj)
+
+
+# ---------------------------------------------------------------------------
+# Workload 3: plain encoder (8192 visible Bernoulli(0.1) bits -> 256 hidden
+# bits = sign of a fixed Gaussian projection of x - 0.1, 3% bit flips).
+# ---------------------------------------------------------------------------
+
+def projection_codes_task(seed, n=500_000, visible=8192, hidden=256, density=0.1, flip=0.03, block=4096):
+    rng = np.random.default_rng(seed)
+    w = np.random.default_rng(seed ^ 0xC0DE).standard_normal((visible, hidden)).astype(np.float32)
+    rows = np.empty((n, (2 * visible + 7) // 8), np.uint8)
+    codes = np.empty((n, (hidden + 7) // 8), np.uint8)
+    for s in range(0, n, block):
+        e = min(n, s + block)
+        xb = rng.random((e - s, visible), dtype=np.float32) < density
+        h = (xb.astype(np.float32) - density) @ w > 0
+        h ^= rng.random(h.shape) < flip
+        rows[s:e] = bits_to_cvec(xb)
+        codes[s:e] = codes_from_bits(h)
+    return dict(rows=rows, codes=codes, visible=visible, hidden=hidden)

@@ -1,0 +1,371 @@
+"""Pins of the CPU oracle (oracle/tm_oracle.c) against independent models.
+
+The reference ships no golden vectors for this path, so each oracle function is
+checked against something that does not share its formulation:
+  * Philox4x32-10: the Random123 known-answer vectors.
+  * bit-plane automata: an arithmetic byte-per-literal counter model
+    (saturating +/-1 on an (m+1)-bit value), including the reference's
+    padding-bit behaviour in its masked byte path.
+  * clause evaluation and predict: numpy on unpacked bits.
+  * training (replay): a pure-Python trainer on the counter model, fed the
+    same RNG stream.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def P(**kw):
+    base = dict(classes=2, features=8, clauses=8, clause_tolerance=4, clause_maximum=4, specificity=2.0)
+    base.update(kw)
+    t = base.pop("type_", "classifier")
+    return O.params(t, **base)
+
+
+# ---------------------------------------------------------------------------
+# RNG
+
+def test_philox_random123_kat():
+    assert list(O.philox([0, 0, 0, 0], [0, 0])) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert list(O.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert list(O.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0])) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_lowbias32_is_a_bijection_and_uniform():
+    L = O.lib()
+    xs = np.arange(1 << 16, dtype=np.uint64) * 65537 + 12345
+    hs = np.array([L.orc_lowbias32(int(x) & 0xFFFFFFFF) for x in xs], np.uint64)
+    assert len(np.unique(hs)) == len(hs)
+    counts = np.bincount((hs >> 28).astype(np.int64), minlength=16)
+    chi2 = ((counts - len(hs) / 16) ** 2 / (len(hs) / 16)).sum()
+    assert chi2 < 50   # 15 dof; p < 1e-5 beyond ~44
+
+
+def test_perm_is_a_permutation_sorted_by_key():
+    for n in (1, 2, 17, 1000):
+        p = O.perm(7, 3, n)
+        assert sorted(p.tolist()) == list(range(n))
+    a, b = O.perm(7, 3, 1000), O.perm(7, 4, 1000)
+    assert not np.array_equal(a, b)
+    assert np.array_equal(O.perm(7, 3, 1000), a)
+
+
+def test_literal_draw_range_and_uniformity():
+    L = O.lib()
+    hits = np.zeros(10, np.int64)
+    for r in range(20000):
+        b = L.orc_literal_draw(0xDEADBEEF, r, 10)
+        assert 0 <= b < 10
+        hits[b] += 1
+    assert hits.min() > 1800 and hits.max() < 2200
+
+
+def test_threshold32():
+    L = O.lib()
+    assert L.orc_threshold32(0.0) == 0
+    assert L.orc_threshold32(-1.0) == 0
+    assert L.orc_threshold32(1.0) == 1 << 32
+    assert L.orc_threshold32(0.75) == 3 << 30
+
+
+# ---------------------------------------------------------------------------
+# derived parameters (capi.c:473-493)
+
+def test_derive_sizes_and_target():
+    p = P(classes=4, features=256, clauses=250, clause_tolerance=64, target=-1.0)
+    assert p.clauses == 256 and p.clause_chunks == 32 and p.input_chunks == 64 and p.tail_mask == 0xFF
+    assert p.target == int(np.sqrt(256 / 2) * 64)
+    assert p.action_chunks == 4 * 256 * 64 and p.state_chunks == 4 * 256 * 7 * 64
+    assert abs(p.negative - 0.25) < 1e-15
+    assert P(target=64.0).target == 64
+    assert P(target=2.5).target == 3
+    assert P(clauses=16, target=0.3).target == 5          # ceil(max(1, 4.8))
+    assert P(clauses=16, target=0.01).target == 1
+    assert P(features=5).tail_mask == 0x03 and P(features=5).input_chunks == 2
+    assert P(features=3).tail_mask == 0x3F and P(features=3).input_chunks == 1
+    assert P(negative=0.5).negative == 0.5
+
+
+# ---------------------------------------------------------------------------
+# counter model
+
+def planes_to_values(p, counts, actions):
+    """(clauses, 8*B) int values of the (m+1)-bit counters."""
+    m, B = p.state_bits - 1, p.input_chunks
+    nc = len(actions) // B
+    cb = np.unpackbits(counts.reshape(nc, m, B), axis=2, bitorder="little").astype(np.int64)
+    ab = np.unpackbits(actions.reshape(nc, B), axis=1, bitorder="little").astype(np.int64)
+    v = ab << m
+    for b in range(m):
+        v |= cb[:, b, :] << b
+    return v
+
+
+def values_to_planes(p, v):
+    m, B = p.state_bits - 1, p.input_chunks
+    nc = v.shape[0]
+    counts = np.zeros((nc, m, B), np.uint8)
+    for b in range(m):
+        counts[:, b, :] = np.packbits(((v >> b) & 1).astype(np.uint8), axis=1, bitorder="little")
+    actions = np.packbits(((v >> m) & 1).astype(np.uint8), axis=1, bitorder="little")
+    return counts.reshape(-1), actions.reshape(-1)
+
+
+def model_step(p, v, active, delta):
+    """Saturating +/-1 on active literals; padding literals of a masked tail
+    byte (reference byte path, input_chunks % 16 != 0) keep their include bit
+    and wrap their m count bits (automata.h:97,106,125,135)."""
+    m = p.state_bits - 1
+    vmax = (1 << (m + 1)) - 1
+    B = p.input_chunks
+    v = v.copy()
+    nbits = 8 * B
+    pad = np.zeros(nbits, bool)
+    if B % 16 != 0:
+        pad[p.input_bits:] = True
+    act = active.astype(bool)
+    norm = act & ~pad
+    v[norm] = np.clip(v[norm] + delta, 0, vmax)
+    pm = act & pad
+    hi = v[pm] & (1 << m)
+    v[pm] = hi | ((v[pm] + delta) % (1 << m))
+    return v
+
+
+def included(p, v):
+    m = p.state_bits - 1
+    ib = min(p.include_bits, m + 1)
+    return (v >> (m + 1 - ib)) == (1 << ib) - 1
+
+
+@pytest.mark.parametrize("features", [3, 5, 8, 63, 64, 100])
+@pytest.mark.parametrize("include_bits", [1, 3])
+def test_counter_ops_match_model(features, include_bits):
+    rng = np.random.default_rng(features * 10 + include_bits)
+    p = P(features=features, include_bits=include_bits)
+    L = O.lib()
+    B = p.input_chunks
+    m = p.state_bits - 1
+    for trial in range(30):
+        v = rng.integers(0, 1 << (m + 1), size=(1, 8 * B))
+        v[0, rng.random(8 * B) < 0.2] = (1 << (m + 1)) - 1
+        v[0, rng.random(8 * B) < 0.2] = 0
+        v[0, rng.random(8 * B) < 0.2] = (1 << m) - 1
+        counts, actions = values_to_planes(p, v)
+        counts = np.ascontiguousarray(np.concatenate([counts, np.zeros(len(counts), np.uint8)]))
+        actions = np.ascontiguousarray(np.concatenate([actions, np.zeros(len(actions), np.uint8)]))
+        xbits = rng.random(8 * B) < 0.5
+        xbits[p.input_bits:] = False
+        x = np.packbits(xbits.astype(np.uint8), bitorder="little")
+        op = trial % 4
+        if op == 0:
+            L.orc_inc(ctypes.byref(p), O.ptr(counts), O.ptr(actions), 0, O.ptr(x))
+            want = model_step(p, v[0], xbits, +1)
+        elif op == 1:
+            L.orc_dec_not_excluded(ctypes.byref(p), O.ptr(counts), O.ptr(actions), 0, O.ptr(x))
+            want = model_step(p, v[0], ~xbits & ~included(p, v[0]), -1)
+        elif op == 2:
+            L.orc_inc_not_excluded(ctypes.byref(p), O.ptr(counts), O.ptr(actions), 0, O.ptr(x))
+            want = model_step(p, v[0], ~xbits & ~included(p, v[0]), +1)
+        else:
+            bit = int(rng.integers(0, p.input_bits))
+            L.orc_dec_byte(ctypes.byref(p), O.ptr(counts), O.ptr(actions), 0, bit // 8, 1 << (bit % 8))
+            a = np.zeros(8 * B, bool)
+            a[bit] = True
+            want = model_step(p, v[0], a, -1)
+        got = planes_to_values(p, counts[: m * B], actions[:B])[0]
+        assert np.array_equal(got, want), (features, include_bits, op)
+
+
+def test_setup_value():
+    p = P(features=5)
+    m = O.Machine(p)
+    m.actions[:] = 0xAB
+    m.setup()
+    v = planes_to_values(p, m.counts, m.actions)
+    assert (v == (1 << (p.state_bits - 1)) - 1).all()
+
+
+# ---------------------------------------------------------------------------
+# clause evaluation and predict (capi.c:85-161, :903-995)
+
+def np_eval(p, actions, rows):
+    """(n, clauses) votes and outputs from unpacked bits."""
+    B = p.input_chunks
+    nc = len(actions) // B
+    a = np.unpackbits(actions.reshape(nc, B), axis=1, bitorder="little")[:, :p.input_bits].astype(np.int64)
+    x = np.unpackbits(rows, axis=1, bitorder="little")[:, :p.input_bits].astype(np.int64)
+    lit = a.sum(1)
+    fail = (1 - x) @ a.T
+    tol = p.clause_tolerance
+    votes = np.where(lit[None, :] == 0, tol, np.maximum(0, np.minimum(lit, tol)[None, :] - fail))
+    return votes
+
+
+def random_machine(p, rng, density=0.05):
+    m = O.Machine(p)
+    m.setup()
+    m.actions[:] = np.packbits((rng.random(len(m.actions) * 8) < density).astype(np.uint8), bitorder="little")
+    return m
+
+
+@pytest.mark.parametrize("features,classes,kind", [(5, 3, "classifier"), (64, 4, "classifier"), (37, 9, "encoder")])
+def test_predict_matches_numpy(features, classes, kind):
+    rng = np.random.default_rng(features)
+    p = P(features=features, classes=classes, clauses=24, clause_tolerance=3, type_=kind)
+    m = random_machine(p, rng, 0.08)
+    xb = rng.random((200, features)) < 0.5
+    from santoku.tsetlin.data import bits_to_cvec
+    rows = bits_to_cvec(xb)
+    votes = np_eval(p, m.actions, rows)
+    sign = np.where(np.arange(votes.shape[1]) % 2 == 0, 1, -1)
+    sums = (votes * sign).reshape(200, classes, p.clauses).sum(2)
+    assert np.array_equal(m.class_sums(rows), sums)
+    out = m.predict(rows)
+    if kind == "classifier":
+        assert np.array_equal(out, sums.argmax(1))
+    else:
+        bits = np.unpackbits(out, axis=1, bitorder="little")[:, :classes]
+        assert np.array_equal(bits, (sums > 0).astype(np.uint8))
+
+
+def test_predict_special_cases():
+    p = P(features=16, classes=5, clauses=16, clause_tolerance=7)
+    m = O.Machine(p)
+    m.setup()                                   # all clauses empty: votes +tol, sums 0
+    rows = np.random.default_rng(0).integers(0, 256, size=(10, p.input_chunks), dtype=np.uint8)
+    assert (m.class_sums(rows) == 0).all()
+    assert (m.predict(rows) == 0).all()          # ties go to the lowest class
+    # one clause including literal 3 (x3): votes 1 iff x3
+    m.actions[:] = 0xFF                          # every other clause fails often
+    m.actions[:] = 0
+    m.actions[0] = 0b1000                        # clause 0 of class 0 includes x3
+    x = np.zeros((2, p.input_chunks), np.uint8)
+    x[0, 0] = 0b1000
+    sums = m.class_sums(x)
+    # clause 0 votes 1 if x3 else 0; the other 15 empty clauses of class 0 cancel except clause 0's partner
+    assert sums[0, 0] - sums[1, 0] == 1
+
+
+# ---------------------------------------------------------------------------
+# replay training against a pure-Python trainer on the counter model
+
+def py_train(p, rows, labels, codes, epochs):
+    L = O.lib()
+    m = p.state_bits - 1
+    B = p.input_chunks
+    nclauses = p.classes * p.clauses
+    v = np.full((nclauses, 8 * B), (1 << m) - 1, np.int64)
+    xs = np.unpackbits(rows, axis=1, bitorder="little").astype(bool)
+    n = rows.shape[0]
+    T = p.target
+    tol = p.clause_tolerance
+    thr = L.orc_threshold32(1.0 - p.negative)
+    s_draws = L.orc_s_draws(ctypes.byref(p))
+    for e in range(epochs):
+        perm = O.perm(p.seed, e, n)
+        ek = O.epoch_keys(p.seed, e)
+        ekp = O.ptr(ek, O._U32P)
+        for ch in range(p.classes * p.clause_chunks):
+            cls = ch // p.clause_chunks
+            for s in perm:
+                s = int(s)
+                x = xs[s]
+                if p.type == 0:
+                    tgt = (int(labels[s]) & 0xFFFFFFFF) == cls
+                    if not tgt and L.orc_skip_draw(ekp, s, ch) < thr:
+                        continue
+                else:
+                    tgt = bool((codes[s, cls // 8] >> (cls % 8)) & 1)
+                cl0 = ch * 8
+                a = (v[cl0:cl0 + 8] >> m) & 1
+                lit = a[:, :p.input_bits].sum(1)
+                fail = (a[:, :p.input_bits] & ~x[:p.input_bits]).sum(1)
+                votes = np.where(lit == 0, tol, np.maximum(0, np.minimum(lit, tol) - fail))
+                cv = int(sum(votes[j] if j % 2 == 0 else -votes[j] for j in range(8)))
+                cv = max(-T, min(T, cv))
+                num = T - cv if tgt else T + cv
+                d = np.zeros(4, np.uint32)
+                L.orc_step_draws(p.seed, e, s, ch, O.ptr(d, O._U32P))
+                for j in range(8):
+                    u = (int(d[j >> 1]) >> (16 * (j & 1))) & 0xFFFF
+                    if not (u * 2 * T < (num << 16)):
+                        continue
+                    type1 = (j % 2 == 0) if tgt else (j % 2 == 1)
+                    cl = cl0 + j
+                    if type1 and votes[j] > 0:
+                        if lit[j] <= p.clause_maximum:
+                            v[cl] = model_step(p, v[cl], x, +1)
+                        v[cl] = model_step(p, v[cl], ~x & ~included(p, v[cl]), -1)
+                    elif type1:
+                        kev = L.orc_literal_key(ekp, s, cl)
+                        for r in range(s_draws):
+                            b = L.orc_literal_draw(kev, r, p.input_bits)
+                            hit = np.zeros(8 * B, bool)
+                            hit[b] = True
+                            v[cl] = model_step(p, v[cl], hit, -1)
+                    elif votes[j] > 0:
+                        v[cl] = model_step(p, v[cl], ~x & ~included(p, v[cl]), +1)
+    return v
+
+
+@pytest.mark.parametrize("features,kind,include_bits", [
+    (5, "classifier", 1), (8, "classifier", 2), (63, "classifier", 1), (12, "encoder", 1)])
+def test_replay_matches_python_trainer(features, kind, include_bits):
+    rng = np.random.default_rng(features)
+    classes = 3
+    p = P(features=features, classes=classes, clauses=8, clause_tolerance=3, clause_maximum=4,
+          specificity=1.5, target=4, include_bits=include_bits, seed=11, type_=kind)
+    n = 24
+    xb = rng.random((n, features)) < 0.4
+    from santoku.tsetlin.data import bits_to_cvec
+    rows = bits_to_cvec(xb)
+    labels = rng.integers(0, classes, n).astype(np.int64) if kind == "classifier" else None
+    codes = rng.integers(0, 256, (n, p.class_chunks), dtype=np.uint8) if kind == "encoder" else None
+    m = O.Machine(p)
+    m.train_replay(rows, labels=labels, codes=codes, epochs=3)
+    want = py_train(p, rows, labels, codes, 3)
+    got = planes_to_values(p, m.counts, m.actions)
+    if p.input_chunks % 16 != 0 and p.input_bits % 8:
+        # the counts of padding bits of a masked tail byte are invisible (the
+        # include bit never moves); compare everything that is observable
+        assert np.array_equal(got[:, :p.input_bits], want[:, :p.input_bits])
+        m_ = p.state_bits - 1
+        assert np.array_equal(got[:, p.input_bits:] >> m_, want[:, p.input_bits:] >> m_)
+    else:
+        assert np.array_equal(got, want)
+    assert m.stats[1] + m.stats[2] + m.stats[3] > 0
+
+
+def test_replay_independent_of_thread_count():
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path[:0]=[%r,%r];"
+        "import numpy as np; from oracle import oracle as O; from santoku.tsetlin import data as D;"
+        "d=D.xor_and_task(3, n_train=512, n_test=16);"
+        "p=O.params('classifier',classes=4,features=256,clauses=32,clause_tolerance=8,clause_maximum=8,specificity=8,target=16,seed=5);"
+        "m=O.Machine(p); m.train_replay(d['train_rows'],labels=d['train_y'],epochs=2);"
+        "sys.stdout.write(str(int(np.frombuffer(m.actions.tobytes(),np.uint64).sum()%%(1<<61))))"
+    ) % (os.path.dirname(os.path.dirname(O.__file__)), os.path.join(os.path.dirname(os.path.dirname(O.__file__)), "lua-santoku-tsetlin_amd"))
+    outs = []
+    for t in ("1", "5"):
+        env = dict(os.environ, OMP_NUM_THREADS=t)
+        outs.append(subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, check=True).stdout)
+    assert outs[0] == outs[1]
+
+
+def test_refshape_learns_an_easy_rule():
+    from santoku.tsetlin import data as D
+    d = D.xor_and_task(1, n_train=2048, n_test=512)
+    y = d["x"][:, 3].astype(np.int64)
+    p = O.params("classifier", classes=2, features=256, clauses=64, clause_tolerance=2, clause_maximum=2,
+                 specificity=8, target=16, seed=1)
+    m = O.Machine(p)
+    m.train_refshape(d["train_rows"], labels=y[:2048], epochs=3, threads=4)
+    assert (m.predict(d["test_rows"]) == y[2048:]).mean() > 0.97
