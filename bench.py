@@ -82,19 +82,17 @@ def cpu_baseline(data, threads, budget_s=15.0):
     m.setup()
     rng = np.random.default_rng(0)
     m.actions[:] = np.packbits(rng.random(m.actions.size * 8) < 0.01, bitorder="little")
-    npred = 64
-    out = np.zeros(npred, np.int64)
-    t0 = time.perf_counter()
-    L.orc_predict_classifier_threads(O.ctypes.byref(p), O.ptr(m.actions), O.ptr(data["test_rows"][:npred]),
-                                     npred, O.ptr(out, O._I64P), threads)
-    tp = time.perf_counter() - t0
-    while tp < 2.0 and npred < 20000:
-        npred *= 4
+    test_rows = data["test_rows"]
+    npred = min(64, test_rows.shape[0])
+    while True:
         out = np.zeros(npred, np.int64)
         t0 = time.perf_counter()
-        L.orc_predict_classifier_threads(O.ctypes.byref(p), O.ptr(m.actions), O.ptr(data["test_rows"][:npred]),
+        L.orc_predict_classifier_threads(O.ctypes.byref(p), O.ptr(m.actions), O.ptr(test_rows[:npred]),
                                          npred, O.ptr(out, O._I64P), threads)
         tp = time.perf_counter() - t0
+        if tp >= 2.0 or npred >= test_rows.shape[0]:
+            break
+        npred = min(npred * 4, test_rows.shape[0])
     return dict(train_sps=train_sps, train_sample=limit, train_s=t, predict_sps=npred / tp, predict_sample=npred,
                 predict_s=tp)
 
@@ -107,7 +105,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n-train", type=int, default=200_000)
     ap.add_argument("--n-test", type=int, default=20_000)
+    ap.add_argument("--cpu-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_only:
+        # child process of the CPU leg: no torch / HIP in this process, so the
+        # oracle's OpenMP runtime is the only one loaded
+        from santoku.tsetlin import data as D
+        data = D.sparse_planted_task(7, n_train=args.n_train, n_test=args.n_test)
+        threads = len(os.sched_getaffinity(0))
+        cb = cpu_baseline(data, threads)
+        cb["threads"] = threads
+        print(json.dumps(cb), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the engine has no multi-GPU path in this build "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -197,9 +206,16 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
-        log("cpu baseline: %d threads" % threads)
-        cb = cpu_baseline(data, threads)
+        import subprocess
+        log("cpu baseline: child process")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-only", "--n-train", str(n),
+                            "--n-test", str(ntest)], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("cpu baseline failed (%d): %s" % (r.returncode, r.stderr[-2000:]))
+        cb = json.loads(r.stdout.strip().splitlines()[-1])
+        threads = cb["threads"]
+        log("cpu baseline: %d threads, %.2f train samples/s, %.1f predict samples/s" % (
+            threads, cb["train_sps"], cb["predict_sps"]))
         line["cpu_baseline"] = {
             "value": round(cb["train_sps"], 2),
             "unit": "train samples/s",

@@ -228,28 +228,37 @@ int create_common(const tm_options *o, int type, tm_machine **out)
   return TM_OK;
 }
 
-// Stage host or device rows (n x input_chunks bytes) as u32 words [n][wd].
-int stage_rows(tm_machine *m, const uint8_t *rows, uint32_t n, bool dev, dbuf &bytes_buf, dbuf &words_buf,
-               const uint32_t **words)
+// Stage host or device rows (n x input_chunks bytes) as u32 words [n][stride]
+// (stride >= wd; words past the row are zero).  Rows are used in place when
+// the caller's bytes already have that layout and 16-byte alignment.
+int stage_rows(tm_machine *m, const uint8_t *rows, uint32_t n, bool dev, uint32_t stride, dbuf &bytes_buf,
+               dbuf &words_buf, const uint32_t **words)
 {
   const size_t B = m->input_chunks, nbytes = (size_t) n * B;
-  const bool aligned = (B % 4 == 0);
+  const bool in_place = (B % 4 == 0) && (stride == m->wd);
   if (dev) {
-    if (aligned && ((uintptr_t) rows % 4 == 0)) { *words = (const uint32_t *) rows; return TM_OK; }
-    int rc = words_buf.ensure((size_t) n * m->wd * 4);
+    if (in_place && ((uintptr_t) rows % 16 == 0)) { *words = (const uint32_t *) rows; return TM_OK; }
+    int rc = words_buf.ensure((size_t) n * stride * 4);
     if (rc) return rc;
-    HIP_TRY(tm_launch_repack(rows, n, (uint32_t) B, m->wd, words_buf.as<uint32_t>(), m->stream));
+    HIP_TRY(tm_launch_repack(rows, n, (uint32_t) B, stride, words_buf.as<uint32_t>(), m->stream));
     *words = words_buf.as<uint32_t>();
     return TM_OK;
   }
   int rc = bytes_buf.ensure(nbytes);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(bytes_buf.p, rows, nbytes, hipMemcpyHostToDevice, m->stream));
-  if (aligned) { *words = bytes_buf.as<uint32_t>(); return TM_OK; }
-  if ((rc = words_buf.ensure((size_t) n * m->wd * 4))) return rc;
-  HIP_TRY(tm_launch_repack(bytes_buf.as<uint8_t>(), n, (uint32_t) B, m->wd, words_buf.as<uint32_t>(), m->stream));
+  if (in_place) { *words = bytes_buf.as<uint32_t>(); return TM_OK; }
+  if ((rc = words_buf.ensure((size_t) n * stride * 4))) return rc;
+  HIP_TRY(tm_launch_repack(bytes_buf.as<uint8_t>(), n, (uint32_t) B, stride, words_buf.as<uint32_t>(), m->stream));
   *words = words_buf.as<uint32_t>();
   return TM_OK;
+}
+
+uint32_t train_row_stride(const tm_machine *m)
+{
+  uint32_t lanes = 64, wpl = 1;
+  tm_train_shape(m->wd, &lanes, &wpl);
+  return (m->wd + wpl - 1) / wpl * wpl;
 }
 
 int prepare_predict(tm_machine *m)
@@ -270,7 +279,7 @@ int predict_sums(tm_machine *m, const uint8_t *rows, uint32_t n, bool dev)
   if ((rc = use_device(m))) return rc;
   if ((rc = prepare_predict(m))) return rc;
   const uint32_t *words;
-  if ((rc = stage_rows(m, rows, n, dev, m->p_rows, m->p_words, &words))) return rc;
+  if ((rc = stage_rows(m, rows, n, dev, m->wd, m->p_rows, m->p_words, &words))) return rc;
   if ((rc = m->p_sums.ensure((size_t) n * m->classes * 4))) return rc;
   HIP_TRY(tm_launch_zero(m->p_sums.p, (size_t) n * m->classes * 4, m->stream));
   tm_predict_launch L;
@@ -351,7 +360,8 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
     }
   } else {
     const uint32_t *words;
-    if ((rc = stage_rows(m, rows, n, dev, m->t_rows, m->t_words, &words))) return rc;
+    const uint32_t stride = train_row_stride(m);
+    if ((rc = stage_rows(m, rows, n, dev, stride, m->t_rows, m->t_words, &words))) return rc;
     const int64_t *dlabels = nullptr;
     const uint8_t *dcodes = nullptr;
     if (want_type == TM_CLASSIFIER) {
@@ -388,6 +398,7 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
     L.ekeys = m->t_ekeys.as<uint32_t>();
     L.n = n;
     L.wd = m->wd;
+    L.row_stride = stride;
     L.clause_chunks = m->clause_chunks;
     L.class_chunks = m->class_chunks;
     L.total_chunks = m->clause_chunks * m->classes;

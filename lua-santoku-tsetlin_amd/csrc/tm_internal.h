@@ -19,6 +19,7 @@ struct tm_train_launch {
   const uint32_t *perm;
   const uint32_t *ekeys;
   uint32_t n, wd;
+  uint32_t row_stride;            // words between rows: >= wd, a multiple of the tile's words per lane
   uint32_t epoch_first, epochs;   // absolute index of the first epoch, count in this launch
   uint32_t clause_chunks, class_chunks, total_chunks;
   uint32_t tolerance, maximum, target, input_bits, s_draws, include_bits, state_bits;
@@ -52,7 +53,7 @@ int tm_launch_prepare_predict(const uint32_t *actions, uint32_t *actT, uint32_t 
 int tm_launch_predict(const tm_predict_launch &L, hipStream_t st);
 int tm_launch_argmax(const int32_t *sums, uint32_t n, uint32_t classes, int64_t *out, hipStream_t st);
 int tm_launch_encode(const int32_t *sums, uint32_t n, uint32_t classes, uint8_t *out, hipStream_t st);
-int tm_launch_repack(const uint8_t *src, uint32_t n, uint32_t bytes, uint32_t wd, uint32_t *dst, hipStream_t st);
+int tm_launch_repack(const uint8_t *src, uint32_t n, uint32_t bytes, uint32_t stride_words, uint32_t *dst, hipStream_t st);
 int tm_launch_zero(void *p, size_t bytes, hipStream_t st);
 // Train kernel tile for a row of wd words: lanes per clause and words per lane.
 int tm_train_shape(uint32_t wd, uint32_t *lanes, uint32_t *words_per_lane);

@@ -21,12 +21,33 @@ namespace {
 // ---------------------------------------------------------------------------
 // helpers
 
+// Sum over the G consecutive lanes of a clause group, returned to every lane
+// of the group.  Full waves reduce with DPP row shifts and row broadcasts
+// (lane 63 ends with the total); smaller groups use butterfly shuffles.
 template <int G>
 __device__ __forceinline__ uint32_t group_sum(uint32_t v)
 {
+  if (G == 64) {
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, true);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xF, 0xF, true);   // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xF, 0xF, true);   // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+  }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
   return v;
+}
+
+// Workgroup barrier that orders LDS only: outstanding global loads (the
+// prefetched literal row) stay in flight across it.
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 template <int G>
@@ -45,56 +66,73 @@ __device__ __forceinline__ uint32_t word_mask(uint32_t w, uint32_t wd, uint32_t 
   return w + 1 < wd ? 0xFFFFFFFFu : (w + 1 == wd ? last : 0u);
 }
 
-// Saturating +1 on the literals set in c, ripple over planes 0..M (plane M is
-// the include plane); overflow ORs every plane back to one (automata.h:88-116,
-// :164-194).
+// Fused saturating +1 on the literals in c and -1 on the literals in b (c and
+// b disjoint): one carry/borrow ripple over planes 0..M, then overflow sets
+// every plane to one and underflow clears every plane (the two ripples of
+// automata.h:88-146 / :164-194 / :556-594 in one pass).
 template <int M, int WPL>
-__device__ __forceinline__ void add_one(uint32_t (&st)[M + 1][WPL], int k, uint32_t c)
+__device__ __forceinline__ void add_sub_one(uint32_t (&st)[M + 1][WPL], int k, uint32_t c, uint32_t b)
 {
 #pragma unroll
   for (int p = 0; p <= M; p++) {
-    uint32_t t = st[p][k] & c;
-    st[p][k] ^= c;
-    c = t;
+    const uint32_t s0 = st[p][k];
+    st[p][k] = s0 ^ c ^ b;
+    c &= s0;
+    b &= ~s0;
   }
 #pragma unroll
-  for (int p = 0; p <= M; p++) st[p][k] |= c;
-}
-
-// Saturating -1 (automata.h:118-146, :556-594).
-template <int M, int WPL>
-__device__ __forceinline__ void sub_one(uint32_t (&st)[M + 1][WPL], int k, uint32_t b)
-{
-#pragma unroll
-  for (int p = 0; p <= M; p++) {
-    uint32_t t = ~st[p][k] & b;
-    st[p][k] ^= b;
-    b = t;
-  }
-#pragma unroll
-  for (int p = 0; p <= M; p++) st[p][k] &= ~b;
+  for (int p = 0; p <= M; p++) st[p][k] = (st[p][k] | c) & ~b;
 }
 
 // include plane AND the top include_bits-1 count planes (automata.h:499-512).
-template <int M, int WPL>
+// IB1: include_bits == 1, where "included" is the include plane itself.
+template <int M, int WPL, bool IB1>
 __device__ __forceinline__ uint32_t included(const uint32_t (&st)[M + 1][WPL], int k, uint32_t ib)
 {
   uint32_t v = st[M][k];
+  if (IB1) return v;
 #pragma unroll
   for (int i = 1; i <= M; i++)
     if ((uint32_t) i < ib) v &= st[M - i][k];
   return v;
 }
 
+// Load WPL consecutive words of a literal row (the row stride is a multiple
+// of WPL, so a lane's words are all inside the row or all past its end).
+template <int WPL>
+__device__ __forceinline__ void load_words(const uint32_t *row, uint32_t wbase, uint32_t wd, uint32_t (&x)[WPL])
+{
+  if (wbase >= wd) {
+#pragma unroll
+    for (int k = 0; k < WPL; k++) x[k] = 0;
+    return;
+  }
+  if constexpr (WPL % 4 == 0) {
+#pragma unroll
+    for (int k = 0; k < WPL; k += 4) {
+      const uint4 v = *reinterpret_cast<const uint4 *>(row + wbase + k);
+      x[k] = v.x; x[k + 1] = v.y; x[k + 2] = v.z; x[k + 3] = v.w;
+    }
+  } else if constexpr (WPL == 2) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(row + wbase);
+    x[0] = v.x; x[1] = v.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < WPL; k++) x[k] = row[wbase + k];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // training
 
-template <int WPL, int G, int M>
+template <int WPL, int G, int M, bool IB1>
 __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
 {
   constexpr int TW = G * WPL;                  // words per clause tile
   const uint32_t tid = threadIdx.x;
-  const uint32_t j = tid / G;                  // clause within the chunk
+  // clause within the chunk; a full-wave group makes it wave-uniform, and
+  // saying so keeps every per-clause decision on the scalar unit
+  const uint32_t j = G == 64 ? __builtin_amdgcn_readfirstlane(tid / G) : tid / G;
   const uint32_t g = tid % G;                  // lane within the clause group
   const uint32_t lane = tid & 63;
   const uint32_t ch = blockIdx.x;
@@ -133,7 +171,7 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
     return group_sum<G>(c);
   };
   uint32_t lit = recount();
-  __syncthreads();
+  lds_barrier();
 
   const uint32_t T = L.target;
   const uint32_t tol = L.tolerance;
@@ -145,123 +183,170 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
     const uint32_t *perm = L.perm + (size_t) e * L.n;
     const uint32_t ek0 = L.ekeys[4 * e + 0], ek1 = L.ekeys[4 * e + 1];
     const uint32_t ek2 = L.ekeys[4 * e + 2], ek3 = L.ekeys[4 * e + 3];
-    for (uint32_t b0 = 0; b0 < L.n; b0 += 64) {
-      const uint32_t pos = b0 + lane;
-      const bool valid = pos < L.n;
-      const uint32_t s = valid ? perm[pos] : 0u;
-      bool tgt, want;
-      if (L.encoder) {
-        tgt = valid && ((L.codes[(size_t) s * L.class_chunks + (cls >> 3)] >> (cls & 7)) & 1);
-        want = valid;
-      } else {
-        const uint32_t sc = valid ? (uint32_t) L.labels[s] : 0xFFFFFFFFu;
-        tgt = valid && sc == cls;
-        want = valid && (tgt || (uint64_t) tm_skip_draw(ek0, ek1, s, ch) >= L.skip_threshold);
-      }
-      uint64_t mask = __ballot(want);
-      if (!mask) continue;
-      const uint64_t tmask = __ballot(tgt);
-      const tm_u4 d = tm_philox(s, ch, epoch, TM_TAG_STEP, L.seed_lo, L.seed_hi);
-      while (mask) {
-        const int q = __builtin_ctzll(mask);
-        mask &= mask - 1;
-        const uint32_t sq = __builtin_amdgcn_readlane(s, q);
-        const bool tq = (tmask >> q) & 1;
-        const uint32_t d0 = __builtin_amdgcn_readlane(d.x, q), d1 = __builtin_amdgcn_readlane(d.y, q);
-        const uint32_t d2 = __builtin_amdgcn_readlane(d.z, q), d3 = __builtin_amdgcn_readlane(d.w, q);
-        const uint32_t dsel = (j >> 1) == 0 ? d0 : (j >> 1) == 1 ? d1 : (j >> 1) == 2 ? d2 : d3;
-        const uint32_t u16 = (dsel >> (16 * (j & 1))) & 0xFFFFu;
 
-        uint32_t x[WPL];
-        const uint32_t *row = L.rows + (size_t) sq * wd;
-#pragma unroll
-        for (int k = 0; k < WPL; k++) x[k] = (wbase + k < wd) ? row[wbase + k] : 0u;
-
-        uint32_t f = 0;
-#pragma unroll
-        for (int k = 0; k < WPL; k++) f += __popc(st[M][k] & ~x[k] & word_mask(wbase + k, wd, L.last_eval));
-        f = group_sum<G>(f);
-        int32_t votes;
-        if (lit == 0) votes = (int32_t) tol;
-        else {
-          votes = (int32_t)(lit < tol ? lit : tol) - (int32_t) f;
-          if (votes < 0) votes = 0;
+    // Sample positions are scanned in blocks of 64, one per lane: lane l holds
+    // the sample id, its target flag and its four step-draw words.  `want`
+    // marks the samples this chunk processes (negative sampling applied).
+    uint32_t b0 = 0, bs = 0;
+    uint64_t want = 0, tmask = 0;
+    tm_u4 bd = { 0, 0, 0, 0 };
+    auto scan = [&]() -> bool {       // advance to the next block with work
+      while (b0 < L.n) {
+        const uint32_t pos = b0 + lane;
+        const bool valid = pos < L.n;
+        const uint32_t s = valid ? perm[pos] : 0u;
+        bool tgt, w;
+        if (L.encoder) {
+          tgt = valid && ((L.codes[(size_t) s * L.class_chunks + (cls >> 3)] >> (cls & 7)) & 1);
+          w = valid;
+        } else {
+          const uint32_t sc = valid ? (uint32_t) L.labels[s] : 0xFFFFFFFFu;
+          tgt = valid && sc == cls;
+          w = valid && (tgt || (uint64_t) tm_skip_draw(ek0, ek1, s, ch) >= L.skip_threshold);
         }
-        if (g == 0) votes_lds[buf][j] = (j & 1) ? -votes : votes;
-        __syncthreads();
-        int32_t v = 0;
+        b0 += 64;
+        want = __ballot(w);
+        if (want) {
+          tmask = __ballot(tgt);
+          bs = s;
+          bd = tm_philox(s, ch, epoch, TM_TAG_STEP, L.seed_lo, L.seed_hi);
+          return true;
+        }
+      }
+      return false;
+    };
+    // take the next wanted sample of the current block (want != 0)
+    uint32_t nq_s = 0, nq_d = 0;
+    bool nq_t = false;
+    auto take = [&]() {
+      const int q = __builtin_ctzll(want);
+      want &= want - 1;
+      nq_s = __builtin_amdgcn_readlane(bs, q);
+      nq_t = (tmask >> q) & 1;
+      const uint32_t d0 = __builtin_amdgcn_readlane(bd.x, q), d1 = __builtin_amdgcn_readlane(bd.y, q);
+      const uint32_t d2 = __builtin_amdgcn_readlane(bd.z, q), d3 = __builtin_amdgcn_readlane(bd.w, q);
+      const uint32_t dsel = (j >> 1) == 0 ? d0 : (j >> 1) == 1 ? d1 : (j >> 1) == 2 ? d2 : d3;
+      nq_d = (dsel >> (16 * (j & 1))) & 0xFFFFu;
+    };
+    uint32_t xn[WPL];
+    auto fetch = [&](uint32_t sample) { load_words<WPL>(L.rows + (size_t) sample * L.row_stride, wbase, wd, xn); };
+
+    if (!scan()) continue;
+    take();
+    fetch(nq_s);
+    for (;;) {
+      const uint32_t sq = nq_s, u16 = nq_d;
+      const bool tq = nq_t;
+      uint32_t x[WPL];
 #pragma unroll
-        for (int c = 0; c < 8; c++) v += votes_lds[buf][c];
-        buf ^= 1;
-        if (v > (int32_t) T) v = (int32_t) T;
-        if (v < -(int32_t) T) v = -(int32_t) T;
-        const uint64_t num = (uint64_t)(tq ? (int64_t) T - v : (int64_t) T + v);
-        const bool fire = (uint64_t) u16 * (2ull * T) < (num << 16);
-        n_steps++;
-        if (!fire) continue;
+      for (int k = 0; k < WPL; k++) x[k] = xn[k];
+      // find the next sample and start loading its row before this one's work
+      const bool more = want ? true : scan();
+      if (more) {
+        take();
+        fetch(nq_s);
+      }
+
+      uint32_t f = 0;
+#pragma unroll
+      for (int k = 0; k < WPL; k++) f += __popc(st[M][k] & ~x[k] & word_mask(wbase + k, wd, L.last_eval));
+      f = group_sum<G>(f);
+      int32_t votes;
+      if (lit == 0) votes = (int32_t) tol;
+      else {
+        votes = (int32_t)(lit < tol ? lit : tol) - (int32_t) f;
+        if (votes < 0) votes = 0;
+      }
+      if (g == 0) votes_lds[buf][j] = (j & 1) ? -votes : votes;
+      lds_barrier();
+      int32_t v = 0;
+#pragma unroll
+      for (int c = 0; c < 8; c++) v += votes_lds[buf][c];
+      v = __builtin_amdgcn_readfirstlane(v);
+      buf ^= 1;
+      if (v > (int32_t) T) v = (int32_t) T;
+      if (v < -(int32_t) T) v = -(int32_t) T;
+      const uint64_t num = (uint64_t)(tq ? (int64_t) T - v : (int64_t) T + v);
+      const bool fire = (uint64_t) u16 * (2ull * T) < (num << 16);
+      n_steps++;
+      if (fire) {
         const bool type1 = tq ? !(j & 1) : (j & 1);
         const bool out = votes > 0;
-        if (type1 && out) {
-          // Type I, clause true: inc true literals (if short enough), then
-          // dec false literals that are not included (capi.c:179-183).
-          const bool grow = lit <= L.maximum;
-          n_t1++;
-          n_grow += grow;
-#pragma unroll
-          for (int k = 0; k < WPL; k++) {
-            const uint32_t live = word_mask(wbase + k, wd, L.last_live);
-            if (grow) add_one<M, WPL>(st, k, x[k] & live);
-            sub_one<M, WPL>(st, k, ~x[k] & ~included<M, WPL>(st, k, L.include_bits) & live);
+        // kind 1: Type I on a true clause (capi.c:179-183): +1 on true
+        //   literals when literals <= clause_maximum, -1 on false literals
+        //   that are not included (the sets are disjoint, so "included" read
+        //   before the +1 is the same as after it);
+        // kind 2: Type I on a false clause (capi.c:184-192): s_draws random
+        //   single-literal decrements;
+        // kind 3: Type II on a true clause (capi.c:194-197): +1 on false
+        //   literals that are not included.
+        const int kind = type1 ? (out ? 1 : 2) : (out ? 3 : 0);
+        if (kind != 0) {
+          const bool grow = kind == 1 && lit <= L.maximum;
+          uint32_t kev = 0, nd = 0, base = 0, pend = 0;
+          if (kind == 1) { n_t1++; n_grow += grow; }
+          else if (kind == 3) n_t2++;
+          else {
+            // The draws are spread over the group's lanes (draw r on lane r % G).
+            // Saturating decrements commute, so each round applies one
+            // decrement to every distinct literal hit by the draws still
+            // pending; a draw whose literal was already hit this round waits
+            // for the next round.
+            kev = tm_literal_key(ek2, ek3, sq, clause);
+            nd = (L.s_draws + G - 1) / G;
+            n_t1z++;
+            n_draws += L.s_draws;
           }
-          lit = recount();
-        } else if (type1) {
-          // Type I, clause false: s_draws random single-literal decrements
-          // (capi.c:184-192).  Saturating decrements commute, so each round
-          // applies one decrement to every distinct literal hit by the draws
-          // still pending; a draw whose literal was already taken this round
-          // waits for the next one.
-          const uint32_t kev = tm_literal_key(ek2, ek3, sq, clause);
-          n_t1z++;
-          n_draws += L.s_draws;
-          const uint32_t nd = (L.s_draws + G - 1) / G;
-          for (uint32_t base = 0; base < nd; base += 32) {
-            const uint32_t cnt = min(32u, nd - base);
-            uint32_t pend = 0;
-            for (uint32_t i = 0; i < cnt; i++)
-              if ((base + i) * G + g < L.s_draws) pend |= 1u << i;
-            while (group_any<G>(pend != 0)) {
+          bool again;
+          do {
+            uint32_t cm[WPL], bm[WPL];
+            if (kind == 2) {
+              if (pend == 0 && base < nd) {
+                const uint32_t cnt = min(32u, nd - base);
+                for (uint32_t i = 0; i < cnt; i++)
+                  if ((base + i) * G + g < L.s_draws) pend |= 1u << i;
+              }
               uint32_t pp = pend;
               while (pp) {
                 const uint32_t i = __builtin_ctz(pp);
                 pp &= pp - 1;
                 const uint32_t lbit = tm_literal_draw(kev, (base + i) * G + g, L.input_bits);
-                const uint32_t bm = 1u << (lbit & 31);
-                const uint32_t old = atomicOr(&hits[j][lbit >> 5], bm);
-                if (!(old & bm)) pend &= ~(1u << i);
+                const uint32_t hb = 1u << (lbit & 31);
+                const uint32_t old = atomicOr(&hits[j][lbit >> 5], hb);
+                if (!(old & hb)) pend &= ~(1u << i);
               }
-              __threadfence_block();
+              __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
 #pragma unroll
               for (int k = 0; k < WPL; k++) {
-                const uint32_t hm = hits[j][wbase + k];
+                cm[k] = 0;
+                bm[k] = hits[j][wbase + k];
                 hits[j][wbase + k] = 0;
-                sub_one<M, WPL>(st, k, hm);
               }
-              __threadfence_block();
-            }
-          }
-          lit = recount();
-        } else if (out) {
-          // Type II, clause true: inc false literals that are not included
-          // (capi.c:194-197).
-          n_t2++;
+              __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
+            } else {
 #pragma unroll
-          for (int k = 0; k < WPL; k++) {
-            const uint32_t live = word_mask(wbase + k, wd, L.last_live);
-            add_one<M, WPL>(st, k, ~x[k] & ~included<M, WPL>(st, k, L.include_bits) & live);
-          }
+              for (int k = 0; k < WPL; k++) {
+                const uint32_t live = word_mask(wbase + k, wd, L.last_live);
+                const uint32_t idle = ~x[k] & ~included<M, WPL, IB1>(st, k, L.include_bits) & live;
+                cm[k] = kind == 3 ? idle : (grow ? x[k] & live : 0u);
+                bm[k] = kind == 1 ? idle : 0u;
+              }
+            }
+            // one update site for every kind keeps the state in place (branch
+            // arms that each rewrite it make the compiler copy all of it)
+#pragma unroll
+            for (int k = 0; k < WPL; k++) add_sub_one<M, WPL>(st, k, cm[k], bm[k]);
+            again = false;
+            if (kind == 2) {
+              const bool pending = group_any<G>(pend != 0);
+              if (!pending) base += 32;
+              again = pending || base < nd;
+            }
+          } while (again);
           lit = recount();
         }
       }
+      if (!more) break;
     }
   }
 
@@ -460,8 +545,11 @@ __global__ void zero_kernel(uint32_t *p, uint64_t nwords)
 template <int WPL, int G>
 int launch_train_shape(const tm_train_launch &L, hipStream_t st)
 {
-  if (L.state_bits != 8) return (int) hipErrorInvalidValue;
-  hipLaunchKernelGGL((train_kernel<WPL, G, 7>), dim3(L.total_chunks), dim3(8 * G), 0, st, L);
+  if (L.state_bits != 8 || L.row_stride % WPL != 0 || L.row_stride < L.wd) return (int) hipErrorInvalidValue;
+  if (L.include_bits <= 1)
+    hipLaunchKernelGGL((train_kernel<WPL, G, 7, true>), dim3(L.total_chunks), dim3(8 * G), 0, st, L);
+  else
+    hipLaunchKernelGGL((train_kernel<WPL, G, 7, false>), dim3(L.total_chunks), dim3(8 * G), 0, st, L);
   return (int) hipGetLastError();
 }
 
