@@ -181,6 +181,45 @@ int tm_last_timing(const tm_machine *m, double *train_ms, double *predict_ms);
 #define TM_NSTATS 6
 int tm_last_train_stats(const tm_machine *m, uint64_t out[TM_NSTATS]);
 
+/* ---------------------------------------------------------------------------
+ * Batched search trials.  No reference counterpart: optimize.lua runs search
+ * trials one after another on one reusable machine (optimize.lua:730-768).
+ * Here N classifiers that share data, classes and features but differ in
+ * clauses, clause_tolerance, clause_maximum, target, specificity,
+ * include_bits and seed train together, one launch per epoch for all of
+ * them (replayed from a captured HIP graph after the first epoch).  Each
+ * trial's automata are bit-identical to a solo tm_train_classifier run with
+ * the same options.  The tile shape and workgroup order are chosen by timing
+ * the candidates on the first epoch (the autotuner); the choice is cached per
+ * (row words, trial count, workgroup count).
+ */
+typedef struct tm_batch tm_batch;
+
+/* Called after each epoch with the 1-based epoch number; active[t] is 1 for
+ * trials still training.  Clearing active[t] freezes trial t (early stop);
+ * returning 0 stops the whole batch, negative reports an error and stops. */
+typedef int (*tm_batch_epoch_fn)(void *user, uint32_t epoch, uint8_t *active);
+
+int tm_batch_create(const tm_options *opts, uint32_t ntrials, tm_batch **out);
+/* Borrowed handle of trial t (valid until tm_batch_destroy): predict,
+ * checkpoint, restore and persist work on it as on any machine. */
+int tm_batch_machine(tm_batch *b, uint32_t trial, tm_machine **out);
+int tm_batch_train_classifier(tm_batch *b, uint32_t n, const uint8_t *problems, const int64_t *solutions,
+                              uint32_t iterations, tm_batch_epoch_fn each, void *user);
+int tm_batch_train_classifier_dev(tm_batch *b, uint32_t n, const uint8_t *problems_dev,
+                                  const int64_t *solutions_dev, uint32_t iterations, tm_batch_epoch_fn each,
+                                  void *user);
+/* Held-out accuracy of every trial on device-resident rows and labels. */
+int tm_batch_accuracy_dev(tm_batch *b, const uint8_t *problems_dev, const int64_t *solutions_dev, uint32_t n,
+                          double *accuracy);
+/* The autotuner's choice: lanes per clause, words per lane, workgroup order
+ * (0 trial-major, 1 heaviest trials first) and its epoch time. */
+int tm_batch_tuning(const tm_batch *b, uint32_t *lanes, uint32_t *wpl, uint32_t *order, double *epoch_ms);
+/* Milliseconds of the last train call inside train kernels and of the last
+ * accuracy call. */
+int tm_batch_timing(const tm_batch *b, double *train_ms, double *eval_ms);
+void tm_batch_destroy(tm_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

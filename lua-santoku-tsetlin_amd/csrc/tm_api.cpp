@@ -14,6 +14,10 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -76,6 +80,7 @@ struct tm_machine {
 
   int device = 0;
   hipStream_t stream = nullptr;
+  bool owns_stream = true;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint32_t wd = 0;                  // u32 words per literal row
   dbuf counts, actions;             // automata state
@@ -179,20 +184,25 @@ int alloc_device(tm_machine *m)
   return TM_OK;
 }
 
-int init_runtime(tm_machine *m)
+int init_runtime(tm_machine *m, hipStream_t shared = nullptr)
 {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess || n == 0) return fail(TM_E_DEVICE, "no HIP device available");
   HIP_TRY(hipGetDevice(&m->device));
-  HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  if (shared) {
+    m->stream = shared;
+    m->owns_stream = false;
+  } else {
+    HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  }
   HIP_TRY(hipEventCreate(&m->ev0));
   HIP_TRY(hipEventCreate(&m->ev1));
   return TM_OK;
 }
 
 // tk_tsetlin_init_classifier, capi.c:449-506.
-int create_common(const tm_options *o, int type, tm_machine **out)
+int create_common(const tm_options *o, int type, tm_machine **out, hipStream_t shared = nullptr)
 {
   const char *what = type == TM_CLASSIFIER ? "create classifier" : "create encoder";
   if (!out) return fail(TM_E_ARG, "%s: out is NULL", what);
@@ -223,7 +233,7 @@ int create_common(const tm_options *o, int type, tm_machine **out)
   m->reusable = o->reusable != 0;
   m->seed = o->seed;
   m->wd = (m->input_chunks + 3) / 4;
-  if ((rc = init_runtime(m)) || (rc = alloc_device(m))) { tm_destroy(m); return rc; }
+  if ((rc = init_runtime(m, shared)) || (rc = alloc_device(m))) { tm_destroy(m); return rc; }
   *out = m;
   return TM_OK;
 }
@@ -309,6 +319,15 @@ int predict_finish_timing(tm_machine *m)
   return TM_OK;
 }
 
+// Enqueue predict of device-resident rows into device output (no sync).
+int predict_enqueue_dev(tm_machine *m, const uint8_t *rows, uint32_t n, int64_t *out_dev)
+{
+  int rc;
+  if ((rc = predict_sums(m, rows, n, true))) return rc;
+  HIP_TRY(tm_launch_argmax(m->p_sums.as<int32_t>(), n, m->classes, out_dev, m->stream));
+  return TM_OK;
+}
+
 int predict_impl(tm_machine *m, const uint8_t *rows, uint32_t n, void *out, bool dev, int want_type)
 {
   int rc;
@@ -389,46 +408,50 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
     HIP_TRY(tm_launch_zero(m->t_stats.p, TM_NSTATS * 8, m->stream));
     tm_train_launch L;
     memset(&L, 0, sizeof L);
-    L.counts = m->counts.as<uint32_t>();
-    L.actions = m->actions.as<uint32_t>();
+    tm_train_trial &P = L.one;
+    P.counts = m->counts.as<uint32_t>();
+    P.actions = m->actions.as<uint32_t>();
+    P.perm = m->t_perm.as<uint32_t>();
+    P.ekeys = m->t_ekeys.as<uint32_t>();
+    P.stats = m->t_stats.as<unsigned long long>();
+    P.clause_chunks = m->clause_chunks;
+    P.tolerance = m->clause_tolerance;
+    P.maximum = m->clause_maximum;
+    P.target = m->target;
+    // capi.c:185: unsigned s = (2 * features) / specificity
+    P.s_draws = (uint32_t)((double)(2u * m->features) / m->specificity);
+    P.include_bits = m->include_bits;
+    double q = 1.0 - m->negative;
+    P.skip_threshold = !(q > 0) ? 0 : (q >= 1 ? (1ull << 32) : (uint64_t) floor(q * 4294967296.0));
+    P.seed_lo = (uint32_t) m->seed;
+    P.seed_hi = (uint32_t)(m->seed >> 32);
+    P.active = 1;
     L.rows = words;
     L.labels = dlabels;
     L.codes = dcodes;
-    L.perm = m->t_perm.as<uint32_t>();
-    L.ekeys = m->t_ekeys.as<uint32_t>();
     L.n = n;
     L.wd = m->wd;
     L.row_stride = stride;
-    L.clause_chunks = m->clause_chunks;
     L.class_chunks = m->class_chunks;
-    L.total_chunks = m->clause_chunks * m->classes;
-    L.tolerance = m->clause_tolerance;
-    L.maximum = m->clause_maximum;
-    L.target = m->target;
+    L.total_blocks = m->clause_chunks * m->classes;
     L.input_bits = m->input_bits;
-    // capi.c:185: unsigned s = (2 * features) / specificity
-    L.s_draws = (uint32_t)((double)(2u * m->features) / m->specificity);
-    L.include_bits = m->include_bits;
     L.state_bits = m->state_bits;
-    double q = 1.0 - m->negative;
-    L.skip_threshold = !(q > 0) ? 0 : (q >= 1 ? (1ull << 32) : (uint64_t) floor(q * 4294967296.0));
-    L.seed_lo = (uint32_t) m->seed;
-    L.seed_hi = (uint32_t)(m->seed >> 32);
     L.encoder = want_type == TM_ENCODER;
     L.last_live = m->last_live_mask();
     L.last_eval = m->last_eval_mask();
-    L.stats = m->t_stats.as<unsigned long long>();
+    L.all_ib1 = m->include_bits <= 1;
     std::vector<uint32_t> ek(4 * group);
     bool stop = false;
     for (uint32_t e0 = 0; e0 < iterations && !stop; e0 += group) {
       const uint32_t cnt = (iterations - e0) < group ? (iterations - e0) : group;
       for (uint32_t e = 0; e < cnt; e++) {
-        tm_u4 k = tm_philox(e0 + e, 0, 0, TM_TAG_EPOCH, L.seed_lo, L.seed_hi);
+        tm_u4 k = tm_philox(e0 + e, 0, 0, TM_TAG_EPOCH, P.seed_lo, P.seed_hi);
         ek[4 * e] = k.x; ek[4 * e + 1] = k.y; ek[4 * e + 2] = k.z; ek[4 * e + 3] = k.w;
       }
       HIP_TRY(hipMemcpyAsync(m->t_ekeys.p, ek.data(), (size_t) cnt * 16, hipMemcpyHostToDevice, m->stream));
       HIP_TRY(tm_launch_perm(m->seed, e0, cnt, n, m->t_perm.as<uint32_t>(), m->t_perm_scratch.p,
                              m->t_perm_scratch.cap, m->stream));
+      P.perm_epoch0 = e0;
       L.epoch_first = e0;
       L.epochs = cnt;
       L.init = e0 == 0;
@@ -759,7 +782,7 @@ void tm_destroy(tm_machine *m)
   for (dbuf *b : bufs) b->release();
   if (m->ev0) hipEventDestroy(m->ev0);
   if (m->ev1) hipEventDestroy(m->ev1);
-  if (m->stream) hipStreamDestroy(m->stream);
+  if (m->stream && m->owns_stream) hipStreamDestroy(m->stream);
   delete m;
 }
 
@@ -790,6 +813,364 @@ int tm_last_timing(const tm_machine *m, double *train_ms, double *predict_ms)
   if (train_ms) *train_ms = m->train_ms;
   if (predict_ms) *predict_ms = m->predict_ms;
   return TM_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched search trials
+
+struct tm_batch {
+  std::vector<tm_machine *> m;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint32_t wd = 0, input_chunks = 0;
+  dbuf trials, block_map, perm, perm_scratch, ekeys, epoch, stats, rows, words, labels, hits, pred;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t lanes = 0, wpl = 0, order = 0;
+  double tuned_ms = 0, train_ms = 0, eval_ms = 0;
+};
+
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::tuple<uint32_t, uint32_t, uint32_t, double>> g_tune_cache;
+
+void batch_release_graph(tm_batch *b)
+{
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Workgroup map (trial, chunk): order 0 is trial-major; order 1 starts with
+// the trials of most random-literal draws per event (the heaviest chunks).
+std::vector<uint32_t> batch_map(const tm_batch *b, const std::vector<tm_train_trial> &T, uint32_t order)
+{
+  std::vector<uint32_t> idx(T.size());
+  for (uint32_t t = 0; t < idx.size(); t++) idx[t] = t;
+  if (order == 1)
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t c) {
+      return (uint64_t) T[a].s_draws * 8 + T[a].clause_chunks > (uint64_t) T[c].s_draws * 8 + T[c].clause_chunks;
+    });
+  std::vector<uint32_t> map;
+  for (uint32_t t : idx) {
+    if (!T[t].active) continue;
+    const uint32_t chunks = T[t].clause_chunks * b->m[t]->classes;
+    for (uint32_t c = 0; c < chunks; c++) { map.push_back(t); map.push_back(c); }
+  }
+  return map;
+}
+
+int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *labels, uint32_t iterations,
+                tm_batch_epoch_fn each, void *user, bool dev)
+{
+  if (!b) return fail(TM_E_ARG, "batch is NULL");
+  if (n > 0 && (!rows || !labels)) return fail(TM_E_ARG, "batch train: problems and solutions must not be NULL");
+  HIP_TRY(hipSetDevice(b->device));
+  const uint32_t nt = (uint32_t) b->m.size();
+  tm_machine *m0 = b->m[0];
+  b->train_ms = 0;
+  for (tm_machine *m : b->m) { m->predict_ready = false; m->trained = false; memset(m->stats, 0, sizeof m->stats); }
+  if (iterations == 0 || n == 0) {
+    for (tm_machine *m : b->m)
+      HIP_TRY(tm_launch_setup(m->counts.as<uint32_t>(), m->actions.as<uint32_t>(), m->nclauses(), m->state_bits - 1,
+                              m->wd, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (tm_machine *m : b->m) m->trained = true;
+    return TM_OK;
+  }
+  // rows: a stride every candidate tile can read
+  uint32_t shapes[16][2];
+  const int nshapes = tm_train_shapes(m0->wd, shapes, 16);
+  if (nshapes <= 0) return fail(TM_E_UNSUPPORTED, "batch train: no train tile covers %u words", m0->wd);
+  uint32_t maxw = 1;
+  for (int i = 0; i < nshapes; i++) maxw = std::max(maxw, shapes[i][1]);
+  const uint32_t stride = (m0->wd + maxw - 1) / maxw * maxw;
+  const uint32_t *words;
+  int rc;
+  if ((rc = stage_rows(m0, rows, n, dev, stride, b->rows, b->words, &words))) return rc;
+  const int64_t *dlabels = labels;
+  if (!dev) {
+    if ((rc = b->labels.ensure((size_t) n * 8))) return rc;
+    HIP_TRY(hipMemcpyAsync(b->labels.p, labels, (size_t) n * 8, hipMemcpyHostToDevice, b->stream));
+    dlabels = b->labels.as<int64_t>();
+  }
+  // per-trial sample orders and epoch keys for every epoch
+  const size_t per = (size_t) iterations * n;
+  if ((rc = b->perm.ensure(per * nt * 4))) return rc;
+  if ((rc = b->perm_scratch.ensure(tm_perm_scratch_bytes(n)))) return rc;
+  if ((rc = b->ekeys.ensure((size_t) nt * iterations * 16))) return rc;
+  if ((rc = b->stats.ensure((size_t) nt * TM_NSTATS * 8))) return rc;
+  if ((rc = b->epoch.ensure(256))) return rc;
+  if ((rc = b->trials.ensure(nt * sizeof(tm_train_trial)))) return rc;
+  std::vector<uint32_t> ek((size_t) nt * iterations * 4);
+  std::vector<tm_train_trial> T(nt);
+  for (uint32_t t = 0; t < nt; t++) {
+    tm_machine *m = b->m[t];
+    if (!m->has_state) return fail(TM_E_STATE, "batch train: trial %u has no automata state", t);
+    uint32_t *pt = b->perm.as<uint32_t>() + per * t;
+    HIP_TRY(tm_launch_perm(m->seed, 0, iterations, n, pt, b->perm_scratch.p, b->perm_scratch.cap, b->stream));
+    for (uint32_t e = 0; e < iterations; e++) {
+      tm_u4 k = tm_philox(e, 0, 0, TM_TAG_EPOCH, (uint32_t) m->seed, (uint32_t)(m->seed >> 32));
+      uint32_t *o = &ek[((size_t) t * iterations + e) * 4];
+      o[0] = k.x; o[1] = k.y; o[2] = k.z; o[3] = k.w;
+    }
+    tm_train_trial &P = T[t];
+    memset(&P, 0, sizeof P);
+    P.counts = m->counts.as<uint32_t>();
+    P.actions = m->actions.as<uint32_t>();
+    P.perm = pt;
+    P.ekeys = b->ekeys.as<uint32_t>() + (size_t) t * iterations * 4;
+    P.stats = nullptr;
+    P.perm_epoch0 = 0;
+    P.clause_chunks = m->clause_chunks;
+    P.tolerance = m->clause_tolerance;
+    P.maximum = m->clause_maximum;
+    P.target = m->target;
+    P.s_draws = (uint32_t)((double)(2u * m->features) / m->specificity);
+    P.include_bits = m->include_bits;
+    const double q = 1.0 - m->negative;
+    P.skip_threshold = !(q > 0) ? 0 : (q >= 1 ? (1ull << 32) : (uint64_t) floor(q * 4294967296.0));
+    P.seed_lo = (uint32_t) m->seed;
+    P.seed_hi = (uint32_t)(m->seed >> 32);
+    P.active = 1;
+  }
+  HIP_TRY(hipMemcpyAsync(b->ekeys.p, ek.data(), ek.size() * 4, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(tm_launch_zero(b->stats.p, (size_t) nt * TM_NSTATS * 8, b->stream));
+  HIP_TRY(tm_launch_zero(b->epoch.p, 256, b->stream));
+
+  tm_train_launch L;
+  memset(&L, 0, sizeof L);
+  L.trials = b->trials.as<tm_train_trial>();
+  L.epoch_dev = b->epoch.as<uint32_t>();
+  L.rows = words;
+  L.labels = dlabels;
+  L.n = n;
+  L.wd = m0->wd;
+  L.row_stride = stride;
+  L.epochs = 1;
+  L.class_chunks = m0->class_chunks;
+  L.input_bits = m0->input_bits;
+  L.state_bits = m0->state_bits;
+  L.last_live = m0->last_live_mask();
+  L.last_eval = m0->last_eval_mask();
+  L.all_ib1 = 1;
+  for (auto &P : T) L.all_ib1 &= P.include_bits <= 1;
+
+  auto upload_map = [&](uint32_t order) -> int {
+    std::vector<uint32_t> map = batch_map(b, T, order);
+    int r = b->block_map.ensure(map.size() * 4 + 8);
+    if (r) return r;
+    HIP_TRY(hipMemcpyAsync(b->block_map.p, map.data(), map.size() * 4, hipMemcpyHostToDevice, b->stream));
+    L.block_map = b->block_map.as<uint32_t>();
+    L.total_blocks = (uint32_t)(map.size() / 2);
+    return TM_OK;
+  };
+
+  // autotune on epoch 0 (every candidate computes the same epoch-0 state)
+  uint64_t nblocks = 0;
+  for (uint32_t t = 0; t < nt; t++) nblocks += (uint64_t) T[t].clause_chunks * b->m[t]->classes;
+  const auto key = std::make_tuple(m0->wd, nt, (uint32_t) nblocks);
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      std::tie(b->lanes, b->wpl, b->order, b->tuned_ms) = it->second;
+      cached = true;
+    }
+  }
+  if (!cached) {
+    HIP_TRY(hipMemcpyAsync(b->trials.p, T.data(), nt * sizeof(tm_train_trial), hipMemcpyHostToDevice, b->stream));
+    double best = 1e300;
+    for (uint32_t order = 0; order < 2; order++) {
+      if ((rc = upload_map(order))) return rc;
+      for (int i = 0; i < nshapes; i++) {
+        L.lanes = shapes[i][0];
+        L.wpl = shapes[i][1];
+        HIP_TRY(hipEventRecord(b->ev0, b->stream));
+        HIP_TRY(tm_launch_train(L, b->stream));
+        HIP_TRY(hipEventRecord(b->ev1, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        if (ms < best) { best = ms; b->lanes = L.lanes; b->wpl = L.wpl; b->order = order; }
+      }
+    }
+    b->tuned_ms = best;
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_cache[key] = std::make_tuple(b->lanes, b->wpl, b->order, b->tuned_ms);
+  }
+  for (uint32_t t = 0; t < nt; t++) T[t].stats = b->stats.as<unsigned long long>() + (size_t) t * TM_NSTATS;
+  HIP_TRY(hipMemcpyAsync(b->trials.p, T.data(), nt * sizeof(tm_train_trial), hipMemcpyHostToDevice, b->stream));
+  if ((rc = upload_map(b->order))) return rc;
+  L.lanes = b->lanes;
+  L.wpl = b->wpl;
+
+  // one epoch = [train kernel, epoch advance], captured once and replayed
+  batch_release_graph(b);
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int lrc = tm_launch_train(L, b->stream);
+  int arc = tm_launch_epoch_advance(b->epoch.as<uint32_t>(), b->stream);
+  hipError_t cap = hipStreamEndCapture(b->stream, &b->graph);
+  if (lrc || arc || cap != hipSuccess) return fail(TM_E_DEVICE, "batch train: graph capture failed (%d %d %d)", lrc, arc, (int) cap);
+  HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+
+  std::vector<uint8_t> active(nt, 1), prev(nt, 1);
+  for (uint32_t e = 0; e < iterations; e++) {
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->train_ms += ms;
+    for (uint32_t t = 0; t < nt; t++) if (active[t]) { b->m[t]->trained = true; b->m[t]->predict_ready = false; }
+    if (!each) continue;
+    int r = each(user, e + 1, active.data());
+    HIP_TRY(hipSetDevice(b->device));
+    if (r < 0) { fprintf(stderr, "Error in epoch callback (status %d)\n", r); break; }
+    if (r == 0) break;
+    bool any = false;
+    for (uint32_t t = 0; t < nt; t++) {
+      if (active[t] != prev[t]) {
+        uint32_t a = active[t] ? 1u : 0u;
+        HIP_TRY(hipMemcpyAsync((char *) b->trials.p + t * sizeof(tm_train_trial) + offsetof(tm_train_trial, active),
+                               &a, 4, hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        prev[t] = active[t];
+      }
+      any |= active[t] != 0;
+    }
+    if (!any) break;
+  }
+  std::vector<uint64_t> st((size_t) nt * TM_NSTATS);
+  HIP_TRY(hipMemcpyAsync(st.data(), b->stats.p, st.size() * 8, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (uint32_t t = 0; t < nt; t++) memcpy(b->m[t]->stats, &st[(size_t) t * TM_NSTATS], TM_NSTATS * 8);
+  return TM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int tm_batch_create(const tm_options *opts, uint32_t ntrials, tm_batch **out)
+{
+  if (!out) return fail(TM_E_ARG, "batch create: out is NULL");
+  *out = nullptr;
+  if (!opts || ntrials == 0) return fail(TM_E_ARG, "batch create: need at least one trial");
+  for (uint32_t t = 1; t < ntrials; t++)
+    if (opts[t].classes != opts[0].classes || opts[t].features != opts[0].features ||
+        opts[t].negative != opts[0].negative || opts[t].state != opts[0].state)
+      return fail(TM_E_ARG, "batch create: trials must share classes, features, negative and state");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(TM_E_DEVICE, "no HIP device available");
+  tm_batch *b = new tm_batch();
+  int rc = TM_OK;
+  if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    tm_batch_destroy(b);
+    return fail(TM_E_DEVICE, "batch create: HIP stream/event creation failed");
+  }
+  for (uint32_t t = 0; t < ntrials && rc == TM_OK; t++) {
+    tm_options o = opts[t];
+    o.reusable = 1;
+    tm_machine *m = nullptr;
+    rc = create_common(&o, TM_CLASSIFIER, &m, b->stream);
+    if (rc == TM_OK) b->m.push_back(m);
+  }
+  if (rc) { tm_batch_destroy(b); return rc; }
+  b->wd = b->m[0]->wd;
+  b->input_chunks = b->m[0]->input_chunks;
+  *out = b;
+  return TM_OK;
+}
+
+int tm_batch_machine(tm_batch *b, uint32_t trial, tm_machine **out)
+{
+  if (!b || !out) return fail(TM_E_ARG, "batch machine: NULL argument");
+  if (trial >= b->m.size()) return fail(TM_E_ARG, "batch machine: trial %u out of range", trial);
+  *out = b->m[trial];
+  return TM_OK;
+}
+
+int tm_batch_train_classifier(tm_batch *b, uint32_t n, const uint8_t *problems, const int64_t *solutions,
+                              uint32_t iterations, tm_batch_epoch_fn each, void *user)
+{
+  return batch_train(b, n, problems, solutions, iterations, each, user, false);
+}
+
+int tm_batch_train_classifier_dev(tm_batch *b, uint32_t n, const uint8_t *problems, const int64_t *solutions,
+                                  uint32_t iterations, tm_batch_epoch_fn each, void *user)
+{
+  return batch_train(b, n, problems, solutions, iterations, each, user, true);
+}
+
+int tm_batch_accuracy_dev(tm_batch *b, const uint8_t *problems, const int64_t *solutions, uint32_t n,
+                          double *accuracy)
+{
+  if (!b || !accuracy) return fail(TM_E_ARG, "batch accuracy: NULL argument");
+  if (n == 0) { for (size_t t = 0; t < b->m.size(); t++) accuracy[t] = 0; return TM_OK; }
+  if (!problems || !solutions) return fail(TM_E_ARG, "batch accuracy: problems and solutions must not be NULL");
+  HIP_TRY(hipSetDevice(b->device));
+  const uint32_t nt = (uint32_t) b->m.size();
+  int rc;
+  if ((rc = b->hits.ensure((size_t) nt * 8))) return rc;
+  if ((rc = b->pred.ensure((size_t) n * 8))) return rc;
+  HIP_TRY(tm_launch_zero(b->hits.p, (size_t) nt * 8, b->stream));
+  HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  for (uint32_t t = 0; t < nt; t++) {
+    if ((rc = predict_enqueue_dev(b->m[t], problems, n, b->pred.as<int64_t>()))) return rc;
+    HIP_TRY(tm_launch_count_hits(b->pred.as<int64_t>(), solutions, n, b->hits.as<unsigned long long>() + t, b->stream));
+  }
+  HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  std::vector<unsigned long long> h(nt);
+  HIP_TRY(hipMemcpyAsync(h.data(), b->hits.p, nt * 8, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->eval_ms = ms;
+  for (uint32_t t = 0; t < nt; t++) accuracy[t] = (double) h[t] / (double) n;
+  return TM_OK;
+}
+
+int tm_batch_tuning(const tm_batch *b, uint32_t *lanes, uint32_t *wpl, uint32_t *order, double *epoch_ms)
+{
+  if (!b) return fail(TM_E_ARG, "batch is NULL");
+  if (lanes) *lanes = b->lanes;
+  if (wpl) *wpl = b->wpl;
+  if (order) *order = b->order;
+  if (epoch_ms) *epoch_ms = b->tuned_ms;
+  return TM_OK;
+}
+
+int tm_batch_timing(const tm_batch *b, double *train_ms, double *eval_ms)
+{
+  if (!b) return fail(TM_E_ARG, "batch is NULL");
+  if (train_ms) *train_ms = b->train_ms;
+  if (eval_ms) *eval_ms = b->eval_ms;
+  return TM_OK;
+}
+
+void tm_batch_destroy(tm_batch *b)
+{
+  if (!b) return;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  batch_release_graph(b);
+  for (tm_machine *m : b->m) tm_destroy(m);
+  dbuf *bufs[] = { &b->trials, &b->block_map, &b->perm, &b->perm_scratch, &b->ekeys, &b->epoch, &b->stats,
+                   &b->rows, &b->words, &b->labels, &b->hits, &b->pred };
+  for (dbuf *x : bufs) x->release();
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
 }
 
 } // extern "C"

@@ -7,28 +7,45 @@
 // Training launch.  Device layouts (u32 words, Wd = ceil(input_chunks/4)):
 //   counts  [clause][plane 0..m-1][Wd]   bit-sliced counters, plane 0 = LSB
 //   actions [clause][Wd]                 the include plane (MSB of each counter)
-//   rows    [n][Wd]                      flip-interleaved literal rows
-//   perm    [epochs][n]                  sample order of each epoch
-//   ekeys   [epochs][4]                  per-epoch key words
-struct tm_train_launch {
+//   rows    [n][row_stride]              flip-interleaved literal rows
+//   perm    [epoch - perm_epoch0][n]     sample order of each epoch
+//   ekeys   [epoch - perm_epoch0][4]     per-epoch key words
+//
+// Per-machine ("trial") parameters live in tm_train_trial; a launch covers one
+// machine (`one`) or a table of machines sharing the data (`trials`, with
+// `block_map` giving each workgroup's (trial, chunk)).
+struct tm_train_trial {
   uint32_t *counts;
   uint32_t *actions;
-  const uint32_t *rows;
-  const int64_t *labels;   // classifier targets (NULL for the encoder)
-  const uint8_t *codes;    // encoder targets [n][class_chunks] (NULL for the classifier)
   const uint32_t *perm;
   const uint32_t *ekeys;
+  unsigned long long *stats;      // optional work counters, TM_NSTATS entries (atomically added)
+  uint64_t skip_threshold;        // negative sampling: skip iff draw < threshold
+  uint32_t perm_epoch0;           // epoch of perm/ekeys row 0
+  uint32_t clause_chunks;
+  uint32_t tolerance, maximum, target, s_draws, include_bits;
+  uint32_t seed_lo, seed_hi;
+  uint32_t active;                // 0: the trial is frozen (its workgroups return at once)
+};
+
+struct tm_train_launch {
+  tm_train_trial one;
+  const tm_train_trial *trials;   // device table, or NULL for `one`
+  const uint32_t *block_map;      // device, 2 words per workgroup: trial, chunk (NULL: trial 0, chunk = block)
+  const uint32_t *epoch_dev;      // if set, the launch's epoch is read here (graph replay); init = epoch == 0
+  const uint32_t *rows;
+  const int64_t *labels;          // classifier targets (NULL for the encoder)
+  const uint8_t *codes;           // encoder targets [n][class_chunks] (NULL for the classifier)
   uint32_t n, wd;
   uint32_t row_stride;            // words between rows: >= wd, a multiple of the tile's words per lane
   uint32_t epoch_first, epochs;   // absolute index of the first epoch, count in this launch
-  uint32_t clause_chunks, class_chunks, total_chunks;
-  uint32_t tolerance, maximum, target, input_bits, s_draws, include_bits, state_bits;
-  uint64_t skip_threshold;        // negative sampling: skip iff draw < threshold
-  uint32_t seed_lo, seed_hi;
+  uint32_t class_chunks, total_blocks;
+  uint32_t input_bits, state_bits;
   uint32_t encoder;
   uint32_t last_live, last_eval;  // masks of the last real word (Wd-1)
   uint32_t init;                  // 1: start from the setup state (capi.c:1136-1146)
-  unsigned long long *stats;      // optional work counters, TM_NSTATS entries (atomically added)
+  uint32_t lanes, wpl;            // tile: lanes per clause, words per lane (0: default shape)
+  uint32_t all_ib1;               // every trial has include_bits == 1
 };
 
 // Work counters of a train launch: TM_NSTATS entries, as documented at
@@ -57,3 +74,9 @@ int tm_launch_repack(const uint8_t *src, uint32_t n, uint32_t bytes, uint32_t st
 int tm_launch_zero(void *p, size_t bytes, hipStream_t st);
 // Train kernel tile for a row of wd words: lanes per clause and words per lane.
 int tm_train_shape(uint32_t wd, uint32_t *lanes, uint32_t *words_per_lane);
+// Every instantiated tile that covers wd words (for the autotuner); returns the count.
+int tm_train_shapes(uint32_t wd, uint32_t (*out)[2], int cap);
+int tm_launch_epoch_advance(uint32_t *epoch_dev, hipStream_t st);
+// hits[t] += number of s < n with pred[s] == labels[s] (one counter per launch)
+int tm_launch_count_hits(const int64_t *pred, const int64_t *labels, uint32_t n, unsigned long long *hits,
+                         hipStream_t st);

@@ -135,11 +135,21 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
   const uint32_t j = G == 64 ? __builtin_amdgcn_readfirstlane(tid / G) : tid / G;
   const uint32_t g = tid % G;                  // lane within the clause group
   const uint32_t lane = tid & 63;
-  const uint32_t ch = blockIdx.x;
-  const uint32_t cls = ch / L.clause_chunks;
+  uint32_t trial = 0, ch = blockIdx.x;
+  if (L.block_map) {
+    trial = L.block_map[2 * blockIdx.x];
+    ch = L.block_map[2 * blockIdx.x + 1];
+  }
+  const tm_train_trial P = L.trials ? L.trials[trial] : L.one;
+  if (!P.active) return;
+  const uint32_t epoch_first = L.epoch_dev ? *L.epoch_dev : L.epoch_first;
+  const bool init = L.epoch_dev ? epoch_first == 0 : L.init != 0;
+  const uint32_t cls = ch / P.clause_chunks;
   const uint32_t clause = ch * 8 + j;
   const uint32_t wbase = g * WPL;
   const uint32_t wd = L.wd;
+  uint32_t *counts = P.counts;
+  uint32_t *actions = P.actions;
 
   __shared__ int32_t votes_lds[2][8];
   __shared__ uint32_t hits[8][TW];
@@ -150,14 +160,14 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
 #pragma unroll
   for (int k = 0; k < WPL; k++) {
     uint32_t w = wbase + k;
-    if (L.init) {
+    if (init) {
 #pragma unroll
       for (int p = 0; p < M; p++) st[p][k] = 0xFFFFFFFFu;
       st[M][k] = 0;
     } else if (w < wd) {
 #pragma unroll
-      for (int p = 0; p < M; p++) st[p][k] = L.counts[((size_t) clause * M + p) * wd + w];
-      st[M][k] = L.actions[(size_t) clause * wd + w];
+      for (int p = 0; p < M; p++) st[p][k] = counts[((size_t) clause * M + p) * wd + w];
+      st[M][k] = actions[(size_t) clause * wd + w];
     } else {
 #pragma unroll
       for (int p = 0; p <= M; p++) st[p][k] = 0;
@@ -173,16 +183,17 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
   uint32_t lit = recount();
   lds_barrier();
 
-  const uint32_t T = L.target;
-  const uint32_t tol = L.tolerance;
+  const uint32_t T = P.target;
+  const uint32_t tol = P.tolerance;
   uint32_t buf = 0;
   uint32_t n_steps = 0, n_t1 = 0, n_grow = 0, n_t2 = 0, n_t1z = 0, n_draws = 0;
 
   for (uint32_t e = 0; e < L.epochs; e++) {
-    const uint32_t epoch = L.epoch_first + e;
-    const uint32_t *perm = L.perm + (size_t) e * L.n;
-    const uint32_t ek0 = L.ekeys[4 * e + 0], ek1 = L.ekeys[4 * e + 1];
-    const uint32_t ek2 = L.ekeys[4 * e + 2], ek3 = L.ekeys[4 * e + 3];
+    const uint32_t epoch = epoch_first + e;
+    const uint32_t er = epoch - P.perm_epoch0;
+    const uint32_t *perm = P.perm + (size_t) er * L.n;
+    const uint32_t ek0 = P.ekeys[4 * er + 0], ek1 = P.ekeys[4 * er + 1];
+    const uint32_t ek2 = P.ekeys[4 * er + 2], ek3 = P.ekeys[4 * er + 3];
 
     // Sample positions are scanned in blocks of 64, one per lane: lane l holds
     // the sample id, its target flag and its four step-draw words.  `want`
@@ -202,14 +213,14 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
         } else {
           const uint32_t sc = valid ? (uint32_t) L.labels[s] : 0xFFFFFFFFu;
           tgt = valid && sc == cls;
-          w = valid && (tgt || (uint64_t) tm_skip_draw(ek0, ek1, s, ch) >= L.skip_threshold);
+          w = valid && (tgt || (uint64_t) tm_skip_draw(ek0, ek1, s, ch) >= P.skip_threshold);
         }
         b0 += 64;
         want = __ballot(w);
         if (want) {
           tmask = __ballot(tgt);
           bs = s;
-          bd = tm_philox(s, ch, epoch, TM_TAG_STEP, L.seed_lo, L.seed_hi);
+          bd = tm_philox(s, ch, epoch, TM_TAG_STEP, P.seed_lo, P.seed_hi);
           return true;
         }
       }
@@ -282,7 +293,7 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
         //   literals that are not included.
         const int kind = type1 ? (out ? 1 : 2) : (out ? 3 : 0);
         if (kind != 0) {
-          const bool grow = kind == 1 && lit <= L.maximum;
+          const bool grow = kind == 1 && lit <= P.maximum;
           uint32_t kev = 0, nd = 0, base = 0, pend = 0;
           if (kind == 1) { n_t1++; n_grow += grow; }
           else if (kind == 3) n_t2++;
@@ -293,18 +304,19 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
             // pending; a draw whose literal was already hit this round waits
             // for the next round.
             kev = tm_literal_key(ek2, ek3, sq, clause);
-            nd = (L.s_draws + G - 1) / G;
+            nd = (P.s_draws + G - 1) / G;
             n_t1z++;
-            n_draws += L.s_draws;
+            n_draws += P.s_draws;
           }
-          bool again;
+          bool again, load = true;   // load: the group starts the batch of draws at `base`
           do {
             uint32_t cm[WPL], bm[WPL];
             if (kind == 2) {
-              if (pend == 0 && base < nd) {
+              if (load) {
                 const uint32_t cnt = min(32u, nd - base);
                 for (uint32_t i = 0; i < cnt; i++)
-                  if ((base + i) * G + g < L.s_draws) pend |= 1u << i;
+                  if ((base + i) * G + g < P.s_draws) pend |= 1u << i;
+                load = false;
               }
               uint32_t pp = pend;
               while (pp) {
@@ -327,7 +339,7 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
 #pragma unroll
               for (int k = 0; k < WPL; k++) {
                 const uint32_t live = word_mask(wbase + k, wd, L.last_live);
-                const uint32_t idle = ~x[k] & ~included<M, WPL, IB1>(st, k, L.include_bits) & live;
+                const uint32_t idle = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & live;
                 cm[k] = kind == 3 ? idle : (grow ? x[k] & live : 0u);
                 bm[k] = kind == 1 ? idle : 0u;
               }
@@ -338,8 +350,13 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
             for (int k = 0; k < WPL; k++) add_sub_one<M, WPL>(st, k, cm[k], bm[k]);
             again = false;
             if (kind == 2) {
+              // every lane of the group leaves a batch together, so no lane
+              // reloads draws that were already applied
               const bool pending = group_any<G>(pend != 0);
-              if (!pending) base += 32;
+              if (!pending) {
+                base += 32;
+                load = true;
+              }
               again = pending || base < nd;
             }
           } while (again);
@@ -350,13 +367,13 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
     }
   }
 
-  if (L.stats && g == 0) {
-    if (j == 0) atomicAdd(&L.stats[0], (unsigned long long) n_steps);
-    atomicAdd(&L.stats[1], (unsigned long long) n_t1);
-    atomicAdd(&L.stats[2], (unsigned long long) n_grow);
-    atomicAdd(&L.stats[3], (unsigned long long) n_t2);
-    atomicAdd(&L.stats[4], (unsigned long long) n_t1z);
-    atomicAdd(&L.stats[5], (unsigned long long) n_draws);
+  if (P.stats && g == 0) {
+    if (j == 0) atomicAdd(&P.stats[0], (unsigned long long) n_steps);
+    atomicAdd(&P.stats[1], (unsigned long long) n_t1);
+    atomicAdd(&P.stats[2], (unsigned long long) n_grow);
+    atomicAdd(&P.stats[3], (unsigned long long) n_t2);
+    atomicAdd(&P.stats[4], (unsigned long long) n_t1z);
+    atomicAdd(&P.stats[5], (unsigned long long) n_draws);
   }
 
 #pragma unroll
@@ -364,8 +381,8 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
     const uint32_t w = wbase + k;
     if (w < wd) {
 #pragma unroll
-      for (int p = 0; p < M; p++) L.counts[((size_t) clause * M + p) * wd + w] = st[p][k];
-      L.actions[(size_t) clause * wd + w] = st[M][k];
+      for (int p = 0; p < M; p++) counts[((size_t) clause * M + p) * wd + w] = st[p][k];
+      actions[(size_t) clause * wd + w] = st[M][k];
     }
   }
 }
@@ -542,38 +559,69 @@ __global__ void zero_kernel(uint32_t *p, uint64_t nwords)
   for (; i < nwords; i += stride) p[i] = 0;
 }
 
+__global__ void count_hits_kernel(const int64_t *pred, const int64_t *labels, uint32_t n, unsigned long long *hits)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool hit = i < n && pred[i] == labels[i];
+  const uint64_t b = __ballot(hit);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(hits, (unsigned long long) __popcll(b));
+}
+
+__global__ void epoch_advance_kernel(uint32_t *epoch)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *epoch += 1;
+}
+
 template <int WPL, int G>
 int launch_train_shape(const tm_train_launch &L, hipStream_t st)
 {
-  if (L.state_bits != 8 || L.row_stride % WPL != 0 || L.row_stride < L.wd) return (int) hipErrorInvalidValue;
-  if (L.include_bits <= 1)
-    hipLaunchKernelGGL((train_kernel<WPL, G, 7, true>), dim3(L.total_chunks), dim3(8 * G), 0, st, L);
+  if (L.state_bits != 8 || L.row_stride % WPL != 0 || L.row_stride < L.wd || (uint64_t) G * WPL < L.wd)
+    return (int) hipErrorInvalidValue;
+  if (L.all_ib1)
+    hipLaunchKernelGGL((train_kernel<WPL, G, 7, true>), dim3(L.total_blocks), dim3(8 * G), 0, st, L);
   else
-    hipLaunchKernelGGL((train_kernel<WPL, G, 7, false>), dim3(L.total_chunks), dim3(8 * G), 0, st, L);
+    hipLaunchKernelGGL((train_kernel<WPL, G, 7, false>), dim3(L.total_blocks), dim3(8 * G), 0, st, L);
   return (int) hipGetLastError();
 }
+
+// instantiated tiles: lanes per clause x words per lane
+const uint32_t kShapes[][2] = { { 8, 1 }, { 8, 2 }, { 16, 1 }, { 16, 2 }, { 32, 1 }, { 32, 2 },
+                                { 64, 1 }, { 64, 2 }, { 64, 4 }, { 64, 8 }, { 64, 16 } };
 
 } // namespace
 
 // ---------------------------------------------------------------------------
 // launchers
 
+// Default tile: the smallest lanes x words that covers the row, one clause
+// group per wave once a row needs 64 lanes.
 int tm_train_shape(uint32_t wd, uint32_t *lanes, uint32_t *wpl)
 {
-  static const uint32_t shapes[][2] = { { 8, 1 }, { 16, 1 }, { 32, 1 }, { 64, 1 }, { 64, 2 }, { 64, 4 }, { 64, 8 }, { 64, 16 } };
-  for (auto &s : shapes)
+  static const uint32_t order[][2] = { { 8, 1 }, { 16, 1 }, { 32, 1 }, { 64, 1 }, { 64, 2 }, { 64, 4 }, { 64, 8 }, { 64, 16 } };
+  for (auto &s : order)
     if (s[0] * s[1] >= wd) { *lanes = s[0]; *wpl = s[1]; return 0; }
   return -1;
 }
 
+int tm_train_shapes(uint32_t wd, uint32_t (*out)[2], int cap)
+{
+  int n = 0;
+  for (auto &s : kShapes)
+    if (s[0] * s[1] >= wd && s[0] * s[1] < 4 * wd + 8 && n < cap) { out[n][0] = s[0]; out[n][1] = s[1]; n++; }
+  return n;
+}
+
 int tm_launch_train(const tm_train_launch &L, hipStream_t st)
 {
-  uint32_t G, W;
-  if (tm_train_shape(L.wd, &G, &W)) return (int) hipErrorInvalidValue;
+  uint32_t G = L.lanes, W = L.wpl;
+  if (!G && tm_train_shape(L.wd, &G, &W)) return (int) hipErrorInvalidValue;
   switch (G * 100 + W) {
     case 801: return launch_train_shape<1, 8>(L, st);
+    case 802: return launch_train_shape<2, 8>(L, st);
     case 1601: return launch_train_shape<1, 16>(L, st);
+    case 1602: return launch_train_shape<2, 16>(L, st);
     case 3201: return launch_train_shape<1, 32>(L, st);
+    case 3202: return launch_train_shape<2, 32>(L, st);
     case 6401: return launch_train_shape<1, 64>(L, st);
     case 6402: return launch_train_shape<2, 64>(L, st);
     case 6404: return launch_train_shape<4, 64>(L, st);
@@ -581,6 +629,19 @@ int tm_launch_train(const tm_train_launch &L, hipStream_t st)
     case 6416: return launch_train_shape<16, 64>(L, st);
   }
   return (int) hipErrorInvalidValue;
+}
+
+int tm_launch_count_hits(const int64_t *pred, const int64_t *labels, uint32_t n, unsigned long long *hits,
+                         hipStream_t st)
+{
+  hipLaunchKernelGGL(count_hits_kernel, dim3((n + 255) / 256), dim3(256), 0, st, pred, labels, n, hits);
+  return (int) hipGetLastError();
+}
+
+int tm_launch_epoch_advance(uint32_t *epoch_dev, hipStream_t st)
+{
+  hipLaunchKernelGGL(epoch_advance_kernel, dim3(1), dim3(64), 0, st, epoch_dev);
+  return (int) hipGetLastError();
 }
 
 int tm_launch_setup(uint32_t *counts, uint32_t *actions, uint64_t nclauses, uint32_t m, uint32_t wd, hipStream_t st)

@@ -1,0 +1,57 @@
+"""santoku.tsetlin.evaluator -- the TM metrics used by the search driver.
+
+Restates tm_class_accuracy (lib/santoku/tsetlin/evaluator/capi.c:119-203) and
+tm_encoding_accuracy (evaluator/capi.c:497-574).  The other evaluator
+functions (retrieval, clustering, dendrograms, bit selection) belong to the
+spectral-hashing pipeline and are not part of this engine.
+"""
+import numpy as np
+
+
+def _as_array(x, dtype):
+    if isinstance(x, (bytes, bytearray, memoryview)):
+        return np.frombuffer(x, dtype=np.uint8).view(dtype)
+    t = type(x)
+    if t.__module__.startswith("torch"):
+        x = x.detach().cpu().numpy()
+    return np.asarray(x).reshape(-1).astype(dtype, copy=False)
+
+
+def class_accuracy(predicted, expected, n_samples, n_classes, threads=None):
+    """Per-class and macro precision/recall/F1 (evaluator/capi.c:119-203).
+    Samples whose predicted or expected class is >= n_classes (after the
+    reference's unsigned conversion) are ignored."""
+    if n_classes == 0:
+        raise ValueError("class_accuracy: n_classes must be > 0")
+    p = _as_array(predicted, np.int64)[:n_samples].astype(np.uint64).astype(np.uint32)
+    e = _as_array(expected, np.int64)[:n_samples].astype(np.uint64).astype(np.uint32)
+    ok = (p < n_classes) & (e < n_classes)
+    p, e = p[ok].astype(np.int64), e[ok].astype(np.int64)
+    hit = p == e
+    tp = np.bincount(e[hit], minlength=n_classes).astype(np.float64)
+    fp = np.bincount(p[~hit], minlength=n_classes).astype(np.float64)
+    fn = np.bincount(e[~hit], minlength=n_classes).astype(np.float64)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        prec = np.where(tp + fp > 0, tp / (tp + fp), 0.0)
+        rec = np.where(tp + fn > 0, tp / (tp + fn), 0.0)
+        f1 = np.where(prec + rec > 0, 2 * prec * rec / (prec + rec), 0.0)
+    classes = [{"precision": float(prec[c]), "recall": float(rec[c]), "f1": float(f1[c])} for c in range(n_classes)]
+    return {"classes": classes, "precision": float(prec.sum() / n_classes), "recall": float(rec.sum() / n_classes),
+            "f1": float(f1.sum() / n_classes)}
+
+
+def encoding_accuracy(predicted, expected, n_samples, n_hidden, threads=None):
+    """Per-bit error rates and their summary (evaluator/capi.c:497-574):
+    bits[j] = fraction of samples whose bit j differs; mean_hamming =
+    1 - mean error; ber_min/ber_max = 1 - min/max per-bit error; ber_std =
+    sample standard deviation of the per-bit errors."""
+    chunks = (n_hidden + 7) // 8
+    pr = _as_array(predicted, np.uint8)[: n_samples * chunks].reshape(n_samples, chunks)
+    ex = _as_array(expected, np.uint8)[: n_samples * chunks].reshape(n_samples, chunks)
+    diff = np.unpackbits(pr ^ ex, axis=1, bitorder="little")[:, :n_hidden]
+    bdiff = diff.sum(0).astype(np.float64)
+    ber = bdiff / n_samples
+    mean = float(bdiff.sum() / (n_samples * n_hidden))
+    std = float(np.sqrt(((ber - mean) ** 2).sum() / (n_hidden - 1))) if n_hidden > 1 else 0.0
+    return {"bits": [float(b) for b in ber], "mean_hamming": 1.0 - mean,
+            "ber_min": 1.0 - float(min(1.0, ber.min())), "ber_max": 1.0 - float(max(0.0, ber.max())), "ber_std": std}

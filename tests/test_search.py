@@ -1,0 +1,115 @@
+"""Evaluator metrics and the search driver (host logic, CPU)."""
+import math
+
+import numpy as np
+import pytest
+
+from santoku.tsetlin import evaluator as ev
+from santoku.tsetlin import optimize as opt
+
+
+def test_class_accuracy_hand_table():
+    # 3 classes; pairs (pred, true)
+    pred = [0, 0, 1, 2, 2, 1, 0, 5]
+    true = [0, 1, 1, 2, 0, 1, 0, 1]      # last sample ignored: class 5 out of range
+    r = ev.class_accuracy(pred, true, 8, 3)
+    # class 0: tp 2 (s0, s6), fp 1 (s1), fn 1 (s4) -> p 2/3, r 2/3
+    # class 1: tp 2 (s2, s5), fp 0, fn 1 (s1) -> p 1, r 2/3
+    # class 2: tp 1 (s3), fp 1 (s4), fn 0 -> p 1/2, r 1
+    f = lambda p, q: 2 * p * q / (p + q)
+    assert r["classes"][0]["precision"] == pytest.approx(2 / 3)
+    assert r["classes"][1]["recall"] == pytest.approx(2 / 3)
+    assert r["classes"][2]["f1"] == pytest.approx(f(0.5, 1.0))
+    assert r["f1"] == pytest.approx((f(2 / 3, 2 / 3) + f(1, 2 / 3) + f(0.5, 1)) / 3)
+    assert r["precision"] == pytest.approx((2 / 3 + 1 + 0.5) / 3)
+
+
+def test_class_accuracy_empty_class_scores_zero():
+    r = ev.class_accuracy([0, 0], [0, 0], 2, 3)
+    assert r["classes"][1] == {"precision": 0.0, "recall": 0.0, "f1": 0.0}
+    assert r["f1"] == pytest.approx(1 / 3)
+
+
+def test_encoding_accuracy_hand_codes():
+    # 4 samples x 10 bits (2 bytes)
+    exp = np.zeros((4, 2), np.uint8)
+    pred = exp.copy()
+    pred[0, 0] = 0b1       # bit 0 wrong in sample 0
+    pred[1, 0] = 0b1       # bit 0 wrong in sample 1
+    pred[2, 1] = 0b10      # bit 9 wrong in sample 2
+    pred[3, 1] = 0b100     # bit 10: padding, not counted
+    r = ev.encoding_accuracy(pred, exp, 4, 10)
+    assert r["bits"][0] == 0.5 and r["bits"][9] == 0.25 and sum(r["bits"]) == 0.75
+    assert r["mean_hamming"] == pytest.approx(1 - 3 / 40)
+    assert r["ber_min"] == 1.0 and r["ber_max"] == 0.5
+    ber = np.array(r["bits"])
+    assert r["ber_std"] == pytest.approx(math.sqrt(((ber - 3 / 40) ** 2).sum() / 9))
+
+
+def test_range_sampler_reflects_rounds_and_clamps():
+    rng = np.random.default_rng(0)
+    s = opt.build_sampler({"def": 16, "min": 8, "max": 256, "round": 8}, 0.2, rng)
+    xs = [s.sample(s.center) for _ in range(2000)]
+    assert all(8 <= x <= 256 and x % 8 == 0 for x in xs)
+    s = opt.build_sampler({"min": 2, "max": 40}, 5.0, rng)      # huge jitter: reflection keeps bounds
+    xs = [s.sample(39.0) for _ in range(2000)]
+    assert all(2 <= x <= 40 for x in xs)
+    s = opt.build_sampler({"min": 64, "max": 4096, "log": True, "pow2": True}, 0.2, rng)
+    xs = {s.sample() for _ in range(500)}
+    assert xs <= {64.0, 128.0, 256.0, 512.0, 1024.0, 2048.0, 4096.0} and len(xs) >= 5
+    s = opt.build_sampler({"min": 1, "max": 4, "int": True}, 0.2, rng)
+    assert {s.sample() for _ in range(400)} == {1, 2, 3, 4}
+    assert opt.build_sampler(7, rng=rng).sample() == 7
+    assert opt.build_sampler([3, 5], rng=rng).sample() in (3, 5)
+    with pytest.raises(ValueError):
+        opt.build_sampler({"def": 3}, rng=rng)
+
+
+def test_log_sampler_is_log_uniform():
+    rng = np.random.default_rng(1)
+    s = opt.build_sampler({"min": 64, "max": 4096, "log": True}, 0.2, rng)
+    xs = np.array([s.sample() for _ in range(20000)])
+    frac_low = (xs < 512).mean()          # log-uniform: P(x < 512) = ln(8)/ln(64) = 0.5
+    assert abs(frac_low - 0.5) < 0.02
+
+
+def test_success_factor_and_jitter_adaptation():
+    assert opt.success_factor(0.3) == 1.0
+    assert opt.success_factor(0.1) == 0.8
+    assert opt.success_factor(0.01) == 0.5
+    rng = np.random.default_rng(2)
+    samplers = opt.build_samplers({"a": {"def": 10, "min": 0, "max": 100}}, ["a"], 0.1, rng)
+    j0 = samplers["a"].state["jitter"]
+    scores = iter([1.0, 0.5, 0.4, 0.3, 0.2])          # 1 improvement in 5 trials -> rate 0.2 -> 0.8
+    res = opt.search(["a"], samplers, lambda p, i: (next(scores), None, None), rounds=1, trials=5, skip_final=True)
+    assert samplers["a"].state["jitter"] == pytest.approx(0.8 * j0)
+    assert samplers["a"].center == res[1]["a"]
+
+
+def test_search_dedups_and_finals():
+    rng = np.random.default_rng(3)
+    samplers = opt.build_samplers({"a": [1, 2]}, ["a"], 0.1, rng)
+    calls = []
+
+    def trial(p, info):
+        calls.append((p["a"], info.get("is_final")))
+        return float(p["a"]), None, "m%d" % p["a"]
+
+    result, best, _ = opt.search(["a"], samplers, trial, rounds=2, trials=6, make_key=lambda p: str(p["a"]))
+    assert best["a"] == 2 and result == "m2"
+    assert calls[-1] == (2, True)
+    assert len(calls) <= 2 * 2 + 1                  # at most 2 distinct keys per round
+
+
+def test_search_batched_uses_proposals():
+    rng = np.random.default_rng(4)
+    samplers = opt.build_samplers({"a": {"def": 5, "min": 0, "max": 10}}, ["a"], 0.2, rng)
+    seen = []
+
+    def propose(s, n, hist):
+        seen.append(len(hist))
+        return [{"a": float(i)} for i in range(n)]
+
+    best, score, hist = opt.search_batched(["a"], samplers, lambda b, info: [p["a"] for p in b], rounds=2, trials=4,
+                                           propose=propose)
+    assert seen == [0, 4] and score == 3.0 and len(hist) == 8
