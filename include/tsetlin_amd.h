@@ -220,6 +220,29 @@ int tm_batch_tuning(const tm_batch *b, uint32_t *lanes, uint32_t *wpl, uint32_t 
 int tm_batch_timing(const tm_batch *b, double *train_ms, double *eval_ms);
 void tm_batch_destroy(tm_batch *b);
 
+/* ---------------------------------------------------------------------------
+ * Search surrogate: an FP32 MLP 12 -> 64 -> 64 -> 64 -> 1 (ReLU hidden layers,
+ * linear output) trained with full-batch mean squared error and Adam by the
+ * engine's own HIP kernels.  No reference counterpart: it predicts a trial's
+ * dev accuracy from its hyper-parameters so the search can propose trials.
+ * Parameters: one float array of TM_MLP_PARAMS values laid out as
+ * W1[64][12] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4.
+ */
+#define TM_MLP_INPUTS 12
+#define TM_MLP_HIDDEN 64
+#define TM_MLP_PARAMS 9217
+typedef struct tm_mlp tm_mlp;
+int tm_mlp_create(const float *params, tm_mlp **out);
+/* `steps` Adam steps (beta 0.9/0.999, eps 1e-8) on n samples X[n][12] -> Y[n];
+ * *loss receives the mean squared error of the last step (may be NULL). */
+int tm_mlp_train(tm_mlp *m, const float *X, const float *Y, uint32_t n, uint32_t steps, float lr, float *loss);
+/* Gradient of the mean squared error at the current parameters (no update). */
+int tm_mlp_gradient(tm_mlp *m, const float *X, const float *Y, uint32_t n, float *grad, float *loss);
+int tm_mlp_predict(tm_mlp *m, const float *X, uint32_t n, float *Y);
+int tm_mlp_params(tm_mlp *m, float *out);
+const char *tm_mlp_last_error(void);
+void tm_mlp_destroy(tm_mlp *m);
+
 #ifdef __cplusplus
 }
 #endif

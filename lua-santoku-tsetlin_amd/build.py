@@ -23,7 +23,7 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["tm_kernels.hip", "tm_api.cpp"]
+SOURCES = ["tm_kernels.hip", "tm_api.cpp", "tm_mlp.hip"]
 
 
 def _compile(src):

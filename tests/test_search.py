@@ -113,3 +113,42 @@ def test_search_batched_uses_proposals():
     best, score, hist = opt.search_batched(["a"], samplers, lambda b, info: [p["a"] for p in b], rounds=2, trials=4,
                                            propose=propose)
     assert seen == [0, 4] and score == 3.0 and len(hist) == 8
+
+
+# ---------------------------------------------------------------------------
+# surrogate MLP oracle (CPU): analytic gradient vs central finite differences
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import mlp_oracle as MO
+    rng = np.random.default_rng(8)
+    p = MO.init(3).astype(np.float64)
+    p += rng.standard_normal(p.size) * 0.01        # break exact zeros of the biases
+    X = rng.standard_normal((40, 12))
+    Y = rng.standard_normal(40)
+    g = MO.gradient(p, X, Y)
+    idx = rng.choice(p.size, 300, replace=False)
+    eps = 1e-6
+    for i in idx:
+        d = np.zeros_like(p)
+        d[i] = eps
+        fd = (MO.loss(p + d, X, Y) - MO.loss(p - d, X, Y)) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-5 + 1e-4 * abs(g[i]), i
+
+
+def test_mlp_oracle_adam_reduces_loss():
+    from oracle import mlp_oracle as MO
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((64, 12))
+    Y = np.sin(X[:, 0]) + 0.5 * X[:, 1] ** 2
+    p0 = MO.init(4)
+    p1, _, _ = MO.adam(p0, X, Y, 200, 3e-3)
+    assert MO.loss(p1, X, Y) < 0.3 * MO.loss(p0, X, Y)
+
+
+def test_surrogate_features_and_target():
+    from santoku.tsetlin import surrogate as S
+    p = dict(clauses=2048, clause_tolerance=64, clause_maximum=64, target=0.125, specificity=10.0, include_bits=1)
+    assert S.effective_target(p) == 256
+    assert S.effective_target(dict(p, target=-1)) == int(np.sqrt(1024) * 64)
+    f = S.features(p, {"epochs": 10, "density": 0.05, "classes": 20, "features": 4096, "best": 0.9})
+    assert len(f) == 12 and f[2] == pytest.approx(0.125) and f[0] == pytest.approx(11 / 12)
