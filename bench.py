@@ -97,6 +97,137 @@ def cpu_baseline(data, threads, budget_s=15.0):
                 predict_s=tp)
 
 
+SEARCH_SPACE = {
+    # workload 4/5: clauses log-uniform in 64..4096 per class, specificity
+    # uniform in 2..40, target clauses/8 or clauses/4; tolerance and maximum
+    # (not fixed by the workload) log-uniform in 1..64
+    "clauses": {"def": 512, "min": 64, "max": 4096, "log": True, "round": 8},
+    "clause_tolerance": {"def": 8, "min": 1, "max": 64, "log": True, "int": True},
+    "clause_maximum": {"def": 16, "min": 1, "max": 64, "log": True, "int": True},
+    "target": [0.125, 0.25],
+    "specificity": {"def": 10, "min": 2, "max": 40},
+    "include_bits": 1,
+}
+
+
+def search_bench(args):
+    """Workloads 4 and 5 on workload 1's data (8192 train / 2048 dev rows, 256
+    features, 4 classes, 10 epochs per trial)."""
+    import numpy as np
+    import torch
+    import santoku.tsetlin as tm
+    from santoku.tsetlin import data as D
+    from santoku.tsetlin import optimize as opt
+    from santoku.tsetlin import surrogate as S
+    from santoku.tsetlin import trials as TR
+
+    d = D.xor_and_task(7)
+    dev = torch.device("cuda:0")
+    rows = torch.from_numpy(d["train_rows"]).to(dev)
+    ys = torch.from_numpy(d["train_y"]).to(dev)
+    drows = torch.from_numpy(d["test_rows"]).to(dev)
+    dys = torch.from_numpy(d["test_y"]).to(dev)
+    n, ndev, epochs = rows.shape[0], drows.shape[0], 10
+    base = dict(classes=4, features=256)
+    rng = np.random.default_rng(args.seed)
+    samplers = opt.build_samplers(SEARCH_SPACE, opt.PARAM_NAMES, 1.0, rng)
+
+    def draw():
+        p = {k: (s.sample(None) if s.type == "range" else s.sample()) for k, s in samplers.items() if s is not None}
+        return opt._swap_tolerance(p)
+
+    trials = [dict(draw(), seed=5000 + i) for i in range(64)]
+    # 64 sequential plain trainings
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solo = []
+    for t in trials:
+        m = tm.classifier(dict(base, **t))
+        m.train(samples=n, problems=rows, solutions=ys, iterations=epochs)
+        solo.append(m)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    log("sequential: 64 trials x %d epochs in %.2f s" % (epochs, t_seq))
+    # the same 64 trials in one batch (first run includes the autotuner)
+    runs = []
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b = TR.Batch(base, trials)
+        b.train(n, rows, ys, epochs)
+        torch.cuda.synchronize()
+        runs.append(time.perf_counter() - t0)
+        log("batched run %d: %.2f s, tuning %s" % (rep, runs[-1], b.tuning()))
+    exact = sum(int(np.array_equal(b.machine(i).checkpoint(), solo[i].checkpoint())) for i in range(64))
+    acc = b.accuracy(drows, dys, ndev)
+    t_batch = runs[-1]
+    tune = b.tuning()
+    del solo
+
+    # surrogate-guided vs plain search: 8 rounds x 64 batched trials
+    ctx = {"epochs": epochs, "density": float(np.unpackbits(d["train_rows"][:, :32]).mean()), "classes": 4,
+           "features": 256}
+
+    def run_search(surrogate, seed):
+        srng = np.random.default_rng(seed)
+        smp = opt.build_samplers(SEARCH_SPACE, opt.PARAM_NAMES, 0.3, srng)
+        fn = TR.batch_trial_fn(base, rows, ys, drows, dys, iterations=epochs, patience=3, seed=seed * 1000)
+        line, best = [], [-1.0]
+        t_start = time.perf_counter()
+
+        def timed(batch, info):
+            sc = fn(batch, info)
+            best[0] = max(best[0], max(sc) if sc else -1.0)
+            line.append((time.perf_counter() - t_start, best[0]))
+            return sc
+
+        prop = S.Proposer(ctx, seed=seed) if surrogate else None
+        _, score, hist = opt.search_batched(opt.PARAM_NAMES, smp, timed, rounds=8, trials=64, propose=prop,
+                                            make_key=opt.make_key)
+        return line, score, (prop.last if prop else None)
+
+    def time_to(line, target):
+        for t, b in line:
+            if b >= target:
+                return t
+        return None
+
+    results = []
+    for seed in (args.seed, args.seed + 1):
+        p_line, p_best, _ = run_search(False, seed)
+        s_line, s_best, s_last = run_search(True, seed)
+        target = p_best - 0.005
+        tp, ts = time_to(p_line, target), time_to(s_line, target)
+        results.append({"seed": seed, "plain_best": round(p_best, 4), "plain_time_to_target_s": round(tp, 2),
+                        "plain_total_s": round(p_line[-1][0], 2), "surrogate_best": round(s_best, 4),
+                        "surrogate_time_to_target_s": None if ts is None else round(ts, 2),
+                        "surrogate_total_s": round(s_line[-1][0], 2),
+                        "ratio_vs_plain_total": None if ts is None else round(ts / p_line[-1][0], 3),
+                        "surrogate_last_round": s_last})
+        log("search seed %d: %s" % (seed, results[-1]))
+    line = {
+        "metric": METRIC,
+        "value": round(64 / t_batch * 3600, 1),
+        "unit": "search trials/hour (batched)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "u32 (automata), f32 (surrogate)",
+        "data": "synthetic workload-1 data (8192 train / 2048 dev rows, 256 features, 4 classes)",
+        "config": {"workload": "64 trials: clauses log-uniform 64..4096/class, specificity 2..40, target "
+                               "clauses/8 or /4, tolerance/maximum log-uniform 1..64; 10 epochs"},
+        "sequential_trials_per_hour": round(64 / t_seq * 3600, 1),
+        "batched_speedup": round(t_seq / t_batch, 2),
+        "batched_first_run_s": round(runs[0], 3),
+        "batched_s": round(t_batch, 3),
+        "sequential_s": round(t_seq, 3),
+        "trials_bitexact_vs_solo": "%d/64" % exact,
+        "autotuner": tune,
+        "dev_accuracy_range": [round(float(acc.min()), 4), round(float(acc.max()), 4)],
+        "search": results,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +237,10 @@ def main():
     ap.add_argument("--n-train", type=int, default=200_000)
     ap.add_argument("--n-test", type=int, default=20_000)
     ap.add_argument("--cpu-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--search", action="store_true", help="batched trials and surrogate search (workloads 4, 5)")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="plain 1 GiB device copies, to scale the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.cpu_only:
         # child process of the CPU leg: no torch / HIP in this process, so the
@@ -121,6 +256,19 @@ def main():
         print("bench.py measures one GPU only: the engine has no multi-GPU path in this build "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.search:
+        search_bench(args)
+        return
+    if args.copy_calibration:
+        import torch
+        src = torch.empty(1 << 28, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": src.numel() * 4, "dispatches": 3}), flush=True)
+        return
 
     import numpy as np
     import torch
