@@ -57,6 +57,7 @@ struct tm_predict_launch {
   const uint32_t *rows;    // [n][Wd]
   int32_t *sums;           // [n][classes]
   uint32_t n, wd, clauses_total, clauses, classes, tolerance, last_eval;
+  uint32_t pass_tiles;     // set by the launcher: 64-clause tiles per L2-resident pass
 };
 
 // Returns a hipError_t.  `ms` receives the time of the train kernels only.

@@ -15,6 +15,7 @@
 #include "tm_internal.h"
 #include "tm_rng.h"
 #include <hipcub/hipcub.hpp>
+#include <algorithm>
 
 namespace {
 
@@ -59,6 +60,17 @@ __device__ __forceinline__ bool group_any(bool pred)
   uint32_t base = lane & ~(uint32_t)(G - 1);
   uint64_t gm = (G == 64) ? ~0ull : ((1ull << G) - 1);
   return ((b >> base) & gm) != 0;
+}
+
+// Workgroups are dealt to the XCDs round-robin (workgroup b runs on XCD b % 8),
+// each XCD with its own L2.  This bijection on [0, nb) gives XCD x a contiguous
+// range of logical indices, so neighbouring tiles (the clause tiles of one
+// predict pass) share an L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t nb)
+{
+  const uint32_t x = b % kXcds, per = nb / kXcds, extra = nb % kXcds;
+  return x * per + min(x, extra) + b / kXcds;
 }
 
 __device__ __forceinline__ uint32_t word_mask(uint32_t w, uint32_t wd, uint32_t last)
@@ -135,6 +147,10 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
   const uint32_t j = G == 64 ? __builtin_amdgcn_readfirstlane(tid / G) : tid / G;
   const uint32_t g = tid % G;                  // lane within the clause group
   const uint32_t lane = tid & 63;
+  // Chunks stay dealt round-robin over the XCDs: each XCD then holds a share
+  // of every class, which balances the per-class work (a chunk's sample count
+  // depends on its class).  Rows are not shared through L2 anyway: every chunk
+  // walks the epoch order at its own pace.
   uint32_t trial = 0, ch = blockIdx.x;
   if (L.block_map) {
     trial = L.block_map[2 * blockIdx.x];
@@ -443,8 +459,25 @@ __global__ __launch_bounds__(64 * PRED_WAVES) void predict_kernel(tm_predict_lau
 {
   __shared__ uint4 xs[PRED_WAVES * PRED_SPW][PRED_WC / 4];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint32_t c = blockIdx.y * 64 + lane;
-  const uint32_t s0 = blockIdx.x * (PRED_WAVES * PRED_SPW);
+  // logical tile order: passes of `pass_tiles` clause tiles; within a pass every
+  // sample tile, and within that the pass's clause tiles (their include plane
+  // stays in the XCD's L2 while the rows stream past)
+  const uint32_t nst = (L.n + PRED_WAVES * PRED_SPW - 1) / (PRED_WAVES * PRED_SPW);
+  const uint32_t nct = (L.clauses_total + 63) / 64;
+  const uint32_t lb = xcd_logical(blockIdx.x, nst * nct);
+  const uint32_t tp = L.pass_tiles, full = nct / tp, in_full = full * tp * nst;
+  uint32_t st_i, ct_i;
+  if (lb < in_full) {
+    const uint32_t pass = lb / (tp * nst), r = lb % (tp * nst);
+    st_i = r / tp;
+    ct_i = pass * tp + r % tp;
+  } else {
+    const uint32_t r = lb - in_full, w = nct - full * tp;
+    st_i = r / w;
+    ct_i = full * tp + r % w;
+  }
+  const uint32_t c = ct_i * 64 + lane;
+  const uint32_t s0 = st_i * (PRED_WAVES * PRED_SPW);
   const bool cvalid = c < L.clauses_total;
   uint32_t acc[PRED_SPW];
 #pragma unroll
@@ -484,7 +517,7 @@ __global__ __launch_bounds__(64 * PRED_WAVES) void predict_kernel(tm_predict_lau
 
   // votes (capi.c:110-119) and class sums (capi.c:928-933)
   const uint32_t lit = cvalid ? L.lit[c] : 0u;
-  const uint32_t cbase = blockIdx.y * 64;
+  const uint32_t cbase = ct_i * 64;
   const uint32_t cls_first = cbase / L.clauses;
   uint32_t clast = cbase + 63;
   if (clast >= L.clauses_total) clast = L.clauses_total - 1;
@@ -700,8 +733,14 @@ int tm_launch_prepare_predict(const uint32_t *actions, uint32_t *actT, uint32_t 
 
 int tm_launch_predict(const tm_predict_launch &L, hipStream_t st)
 {
-  dim3 grid((L.n + PRED_WAVES * PRED_SPW - 1) / (PRED_WAVES * PRED_SPW), (L.clauses_total + 63) / 64);
-  hipLaunchKernelGGL(predict_kernel, grid, dim3(64 * PRED_WAVES), 0, st, L);
+  const uint64_t nst = (L.n + PRED_WAVES * PRED_SPW - 1) / (PRED_WAVES * PRED_SPW);
+  const uint64_t nct = (L.clauses_total + 63) / 64;
+  if (nst * nct > 0x7FFFFFFFull) return (int) hipErrorInvalidValue;
+  tm_predict_launch P = L;
+  // clause tiles per pass: about 2 MiB of include plane
+  const uint64_t tile_bytes = 64ull * L.wd * 4;
+  P.pass_tiles = (uint32_t) std::max<uint64_t>(1, std::min<uint64_t>(nct, (2ull << 20) / tile_bytes));
+  hipLaunchKernelGGL(predict_kernel, dim3((uint32_t)(nst * nct)), dim3(64 * PRED_WAVES), 0, st, P);
   return (int) hipGetLastError();
 }
 
