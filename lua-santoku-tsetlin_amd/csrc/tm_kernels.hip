@@ -78,22 +78,44 @@ __device__ __forceinline__ uint32_t word_mask(uint32_t w, uint32_t wd, uint32_t 
   return w + 1 < wd ? 0xFFFFFFFFu : (w + 1 == wd ? last : 0u);
 }
 
-// Fused saturating +1 on the literals in c and -1 on the literals in b (c and
-// b disjoint): one carry/borrow ripple over planes 0..M, then overflow sets
-// every plane to one and underflow clears every plane (the two ripples of
-// automata.h:88-146 / :164-194 / :556-594 in one pass).
+// Per literal: every plane one (the +1 bound) / some plane 0..top one (not
+// at the -1 bound).
 template <int M, int WPL>
-__device__ __forceinline__ void add_sub_one(uint32_t (&st)[M + 1][WPL], int k, uint32_t c, uint32_t b)
+__device__ __forceinline__ uint32_t all_ones(const uint32_t (&st)[M + 1][WPL], int k)
 {
+  uint32_t v = st[0][k];
+#pragma unroll
+  for (int p = 1; p <= M; p++) v &= st[p][k];
+  return v;
+}
+
+template <int M, int WPL, int TOP>
+__device__ __forceinline__ uint32_t any_one(const uint32_t (&st)[M + 1][WPL], int k)
+{
+  uint32_t v = st[0][k];
+#pragma unroll
+  for (int p = 1; p <= TOP; p++) v |= st[p][k];
+  return v;
+}
+
+// +1 (sgn all ones) or -1 (sgn zero) on the literals in m: the carry / borrow
+// ripple of automata.h:88-146 / :164-194 / :556-594.  The caller has already
+// dropped the literals at the bound (the saturation of those routines), so the
+// ripple never leaves plane M.  Branch-free, so groups of one wave may step in
+// different directions: two logic ops per plane.
+template <int M, int WPL>
+__device__ __forceinline__ void ripple(uint32_t (&st)[M + 1][WPL], int k, uint32_t m, uint32_t sgn)
+{
+  const uint32_t ns = ~sgn;
 #pragma unroll
   for (int p = 0; p <= M; p++) {
     const uint32_t s0 = st[p][k];
-    st[p][k] = s0 ^ c ^ b;
-    c &= s0;
-    b &= ~s0;
+    st[p][k] = s0 ^ m;
+    // m & (s0 ^ ns) as one three-input op (truth table over m, s0, ns:
+    // 0xF0 & (0xCC ^ 0xAA) = 0x60); left to itself the compiler spends an
+    // extra xor per plane here
+    if (p < M) m = __builtin_amdgcn_bitop3_b32(m, s0, ns, 0x60);
   }
-#pragma unroll
-  for (int p = 0; p <= M; p++) st[p][k] = (st[p][k] | c) & ~b;
 }
 
 // include plane AND the top include_bits-1 count planes (automata.h:499-512).
@@ -217,6 +239,7 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
     uint32_t b0 = 0, bs = 0;
     uint64_t want = 0, tmask = 0;
     tm_u4 bd = { 0, 0, 0, 0 };
+    uint32_t bsel = 0;                 // full-wave group: this clause's draw word of bd
     auto scan = [&]() -> bool {       // advance to the next block with work
       while (b0 < L.n) {
         const uint32_t pos = b0 + lane;
@@ -237,6 +260,7 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
           tmask = __ballot(tgt);
           bs = s;
           bd = tm_philox(s, ch, epoch, TM_TAG_STEP, P.seed_lo, P.seed_hi);
+          if constexpr (G == 64) bsel = (j >> 1) == 0 ? bd.x : (j >> 1) == 1 ? bd.y : (j >> 1) == 2 ? bd.z : bd.w;
           return true;
         }
       }
@@ -250,9 +274,14 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
       want &= want - 1;
       nq_s = __builtin_amdgcn_readlane(bs, q);
       nq_t = (tmask >> q) & 1;
-      const uint32_t d0 = __builtin_amdgcn_readlane(bd.x, q), d1 = __builtin_amdgcn_readlane(bd.y, q);
-      const uint32_t d2 = __builtin_amdgcn_readlane(bd.z, q), d3 = __builtin_amdgcn_readlane(bd.w, q);
-      const uint32_t dsel = (j >> 1) == 0 ? d0 : (j >> 1) == 1 ? d1 : (j >> 1) == 2 ? d2 : d3;
+      uint32_t dsel;
+      if constexpr (G == 64) {
+        dsel = __builtin_amdgcn_readlane(bsel, q);
+      } else {
+        const uint32_t d0 = __builtin_amdgcn_readlane(bd.x, q), d1 = __builtin_amdgcn_readlane(bd.y, q);
+        const uint32_t d2 = __builtin_amdgcn_readlane(bd.z, q), d3 = __builtin_amdgcn_readlane(bd.w, q);
+        dsel = (j >> 1) == 0 ? d0 : (j >> 1) == 1 ? d1 : (j >> 1) == 2 ? d2 : d3;
+      }
       nq_d = (dsel >> (16 * (j & 1))) & 0xFFFFu;
     };
     uint32_t xn[WPL];
@@ -325,8 +354,9 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
             n_draws += P.s_draws;
           }
           bool again, load = true;   // load: the group starts the batch of draws at `base`
+          uint32_t phase = 0;
           do {
-            uint32_t cm[WPL], bm[WPL];
+            uint32_t mk[WPL], sgn;
             if (kind == 2) {
               if (load) {
                 const uint32_t cnt = min(32u, nd - base);
@@ -346,24 +376,38 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
               __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
 #pragma unroll
               for (int k = 0; k < WPL; k++) {
-                cm[k] = 0;
-                bm[k] = hits[j][wbase + k];
+                mk[k] = hits[j][wbase + k] & any_one<M, WPL, M>(st, k);
                 hits[j][wbase + k] = 0;
               }
               __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
-            } else {
+              sgn = 0u;
+            } else if (kind == 1 && grow && phase == 0) {
+              // +1 on the true literals first, as tk_automata_inc precedes
+              // tk_automata_dec_not_excluded (capi.c:181-182)
 #pragma unroll
-              for (int k = 0; k < WPL; k++) {
-                const uint32_t live = word_mask(wbase + k, wd, L.last_live);
-                const uint32_t idle = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & live;
-                cm[k] = kind == 3 ? idle : (grow ? x[k] & live : 0u);
-                bm[k] = kind == 1 ? idle : 0u;
-              }
+              for (int k = 0; k < WPL; k++)
+                mk[k] = x[k] & ~all_ones<M, WPL>(st, k) & word_mask(wbase + k, wd, L.last_live);
+              sgn = ~0u;
+            } else if (kind == 3) {
+              // +1 on false literals that are not included; none is at the
+              // bound, since a counter with every plane one is included
+#pragma unroll
+              for (int k = 0; k < WPL; k++)
+                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & word_mask(wbase + k, wd, L.last_live);
+              sgn = ~0u;
+            } else {
+              // -1 on false literals that are not included and not at zero
+              // (with include_bits 1 their action plane is zero already)
+#pragma unroll
+              for (int k = 0; k < WPL; k++)
+                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & word_mask(wbase + k, wd, L.last_live) &
+                        any_one<M, WPL, IB1 ? M - 1 : M>(st, k);
+              sgn = 0u;
             }
             // one update site for every kind keeps the state in place (branch
             // arms that each rewrite it make the compiler copy all of it)
 #pragma unroll
-            for (int k = 0; k < WPL; k++) add_sub_one<M, WPL>(st, k, cm[k], bm[k]);
+            for (int k = 0; k < WPL; k++) ripple<M, WPL>(st, k, mk[k], sgn);
             again = false;
             if (kind == 2) {
               // every lane of the group leaves a batch together, so no lane
@@ -374,9 +418,14 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
                 load = true;
               }
               again = pending || base < nd;
+            } else if (kind == 1 && grow && phase == 0) {
+              again = true;
             }
+            phase++;
           } while (again);
-          lit = recount();
+          // with include_bits 1, a Type I step without growth only lowers
+          // counters whose action bit is zero: the literal count stands
+          if (!(IB1 && kind == 1 && !grow)) lit = recount();
         }
       }
       if (!more) break;
