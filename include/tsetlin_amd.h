@@ -172,6 +172,11 @@ const char *tm_last_error(void);
  * the predict kernels. */
 int tm_last_timing(const tm_machine *m, double *train_ms, double *predict_ms);
 
+/* Milliseconds of the last train call spent building the classifier's
+ * per-epoch step queues (the pre-pass kernels ahead of each train launch;
+ * 0 when none was built).  No reference counterpart. */
+int tm_last_queue_timing(const tm_machine *m, double *queue_ms);
+
 /* Work counters of the last train call, summed over its epochs (no reference
  * counterpart; used to price the kernels against the VALU roofline):
  * [0] chunk-steps evaluated (negative-sampling skips excluded),

@@ -89,8 +89,11 @@ struct tm_machine {
   dbuf t_rows, t_words, t_labels, t_codes, t_perm, t_perm_scratch, t_ekeys;   // training data
   dbuf p_rows, p_words, p_sums, p_out;                                        // predict data
   dbuf t_stats;
+  dbuf q_counts, q_offsets, q_scratch, q_id, q_draw;                          // step queue
+  uint64_t q_cap = 0;
+  hipEvent_t evq0 = nullptr, evq1 = nullptr;
   uint64_t stats[TM_NSTATS] = { 0 };
-  double train_ms = 0, predict_ms = 0;
+  double train_ms = 0, predict_ms = 0, queue_ms = 0;
 
   uint64_t nclauses() const { return (uint64_t) classes * clauses; }
   uint32_t last_eval_mask() const
@@ -198,6 +201,8 @@ int init_runtime(tm_machine *m, hipStream_t shared = nullptr)
   }
   HIP_TRY(hipEventCreate(&m->ev0));
   HIP_TRY(hipEventCreate(&m->ev1));
+  HIP_TRY(hipEventCreate(&m->evq0));
+  HIP_TRY(hipEventCreate(&m->evq1));
   return TM_OK;
 }
 
@@ -350,6 +355,79 @@ int predict_impl(tm_machine *m, const uint8_t *rows, uint32_t n, void *out, bool
   return predict_finish_timing(m);
 }
 
+// The classifier trains from per-epoch step queues unless TSETLIN_AMD_QUEUE=0.
+bool queue_enabled()
+{
+  const char *v = getenv("TSETLIN_AMD_QUEUE");
+  return !(v && v[0] == '0');
+}
+
+// Queue entries beyond this many bytes are not worth their memory: the
+// epoch then runs with the in-kernel scan.
+const uint64_t kQueueMaxBytes = 16ull << 30;
+
+// Build the step queue of one classifier epoch (perm and epoch keys already
+// in place) and point L at it.  *built = false: the queue would not fit, L is
+// left in scan mode.
+int build_queue(tm_machine *m, tm_train_launch &L, const uint32_t ek[4], uint32_t epoch, bool *built)
+{
+  *built = false;
+  L.qoff = nullptr; L.qid = nullptr; L.qdraw = nullptr; L.qcap = 0; L.qseg = 0;
+  const uint32_t nseg = tm_queue_segments(L.n);
+  const uint64_t items = (uint64_t) L.total_blocks * nseg + 1;
+  if (items > 0x7FFFFFFFull) return TM_OK;
+  int rc;
+  if ((rc = m->q_counts.ensure(items * 8)) || (rc = m->q_offsets.ensure(items * 8)) ||
+      (rc = m->q_scratch.ensure(tm_queue_scratch_bytes(items))))
+    return rc;
+  tm_queue_launch Q;
+  memset(&Q, 0, sizeof Q);
+  Q.perm = L.one.perm;
+  Q.labels = L.labels;
+  Q.n = L.n;
+  Q.chunks = L.total_blocks;
+  Q.clause_chunks = L.one.clause_chunks;
+  Q.nseg = nseg;
+  Q.epoch = epoch;
+  Q.ek0 = ek[0];
+  Q.ek1 = ek[1];
+  Q.seed_lo = L.one.seed_lo;
+  Q.seed_hi = L.one.seed_hi;
+  Q.skip_threshold = L.one.skip_threshold;
+  Q.counts = m->q_counts.as<uint64_t>();
+  Q.offsets = m->q_offsets.as<uint64_t>();
+  HIP_TRY(hipEventRecord(m->evq0, m->stream));
+  HIP_TRY(tm_launch_queue_count(Q, m->q_scratch.p, m->q_scratch.cap, m->stream));
+  uint64_t total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, Q.offsets + (items - 1), 8, hipMemcpyDeviceToHost, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  if (total * 20 > kQueueMaxBytes) return TM_OK;
+  if (total > m->q_cap || !m->q_id.p || !m->q_draw.p) {
+    // some headroom: the count moves a little from epoch to epoch
+    const uint64_t cap = std::max<uint64_t>(64, total + total / 8);
+    if ((rc = m->q_id.ensure(cap * 4)) || (rc = m->q_draw.ensure(cap * 16))) {
+      m->q_id.release();
+      m->q_draw.release();
+      m->q_cap = 0;
+      (void) hipGetLastError();
+      return TM_OK;
+    }
+    m->q_cap = cap;
+  }
+  Q.qid = m->q_id.as<uint32_t>();
+  Q.qdraw = m->q_draw.as<uint32_t>();
+  Q.qcap = m->q_cap;
+  HIP_TRY(tm_launch_queue_write(Q, m->stream));
+  HIP_TRY(hipEventRecord(m->evq1, m->stream));
+  L.qoff = Q.offsets;
+  L.qid = Q.qid;
+  L.qdraw = Q.qdraw;
+  L.qcap = Q.qcap;
+  L.qseg = nseg;
+  *built = true;
+  return TM_OK;
+}
+
 // tk_tsetlin_train_classifier / _encoder, capi.c:1112-1282.
 int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *labels, const uint8_t *codes,
                uint32_t iterations, tm_epoch_fn each, void *user, bool dev, int want_type)
@@ -364,6 +442,7 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
   if (n > 0 && want_type == TM_ENCODER && !codes) return fail(TM_E_ARG, "train: codes must not be NULL");
   if ((rc = use_device(m))) return rc;
   m->train_ms = 0;
+  m->queue_ms = 0;
   memset(m->stats, 0, sizeof m->stats);
   m->predict_ready = false;
   if (iterations == 0 || n == 0) {
@@ -399,8 +478,12 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
       }
     }
     // Without a callback, epochs run in groups inside one launch each (the
-    // chunk state stays in registers across the epochs of a group).
-    const uint32_t group = each ? 1u : 8u;
+    // chunk state stays in registers across the epochs of a group).  A
+    // classifier instead builds a step queue per epoch and launches once per
+    // epoch: the queue takes the epoch-order scan, the negative-sampling
+    // draws and the step draws out of the train kernel.
+    const bool queue = want_type == TM_CLASSIFIER && queue_enabled();
+    const uint32_t group = (each || queue) ? 1u : 8u;
     if ((rc = m->t_perm.ensure((size_t) group * n * 4))) return rc;
     if ((rc = m->t_perm_scratch.ensure(tm_perm_scratch_bytes(n)))) return rc;
     if ((rc = m->t_ekeys.ensure((size_t) group * 16))) return rc;
@@ -455,6 +538,8 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
       L.epoch_first = e0;
       L.epochs = cnt;
       L.init = e0 == 0;
+      bool queued = false;
+      if (queue && (rc = build_queue(m, L, ek.data(), e0, &queued))) return rc;
       HIP_TRY(hipEventRecord(m->ev0, m->stream));
       HIP_TRY(tm_launch_train(L, m->stream));
       HIP_TRY(hipEventRecord(m->ev1, m->stream));
@@ -462,6 +547,10 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
       float ms = 0;
       HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
       m->train_ms += ms;
+      if (queued) {
+        HIP_TRY(hipEventElapsedTime(&ms, m->evq0, m->evq1));
+        m->queue_ms += ms;
+      }
       m->predict_ready = false;
       if (each) {
         m->trained = true;
@@ -476,6 +565,10 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
     HIP_TRY(hipMemcpyAsync(m->stats, m->t_stats.p, sizeof m->stats, hipMemcpyDeviceToHost, m->stream));
     HIP_TRY(hipStreamSynchronize(m->stream));
   }
+  // the queue is sized by the data: give its memory back between calls
+  m->q_id.release();
+  m->q_draw.release();
+  m->q_cap = 0;
   // capi.c:1187-1189: a non-reusable machine releases its counters
   if (!m->reusable) {
     m->counts.release();
@@ -778,10 +871,13 @@ void tm_destroy(tm_machine *m)
   if (m->stream) hipStreamSynchronize(m->stream);
   dbuf *bufs[] = { &m->counts, &m->actions, &m->actT, &m->lit, &m->t_rows, &m->t_words, &m->t_labels,
                    &m->t_codes, &m->t_perm, &m->t_perm_scratch, &m->t_ekeys, &m->p_rows, &m->p_words,
-                   &m->p_sums, &m->p_out, &m->t_stats };
+                   &m->p_sums, &m->p_out, &m->t_stats, &m->q_counts, &m->q_offsets, &m->q_scratch,
+                   &m->q_id, &m->q_draw };
   for (dbuf *b : bufs) b->release();
   if (m->ev0) hipEventDestroy(m->ev0);
   if (m->ev1) hipEventDestroy(m->ev1);
+  if (m->evq0) hipEventDestroy(m->evq0);
+  if (m->evq1) hipEventDestroy(m->evq1);
   if (m->stream && m->owns_stream) hipStreamDestroy(m->stream);
   delete m;
 }
@@ -812,6 +908,13 @@ int tm_last_timing(const tm_machine *m, double *train_ms, double *predict_ms)
   if (!m) return fail(TM_E_ARG, "machine is NULL");
   if (train_ms) *train_ms = m->train_ms;
   if (predict_ms) *predict_ms = m->predict_ms;
+  return TM_OK;
+}
+
+int tm_last_queue_timing(const tm_machine *m, double *queue_ms)
+{
+  if (!m) return fail(TM_E_ARG, "machine is NULL");
+  if (queue_ms) *queue_ms = m->queue_ms;
   return TM_OK;
 }
 

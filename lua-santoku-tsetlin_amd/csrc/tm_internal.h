@@ -46,6 +46,28 @@ struct tm_train_launch {
   uint32_t init;                  // 1: start from the setup state (capi.c:1136-1146)
   uint32_t lanes, wpl;            // tile: lanes per clause, words per lane (0: default shape)
   uint32_t all_ib1;               // every trial has include_bits == 1
+  // Step queue (classifier, one machine, one epoch per launch; NULL qid: the
+  // kernel scans the epoch order itself).  Chunk ch trains on entries
+  // [qoff[ch * qseg], qoff[(ch + 1) * qseg]) in order.
+  const uint64_t *qoff;
+  const uint32_t *qid;            // sample | target << 31
+  const uint32_t *qdraw;          // [4][qcap] step-draw words (Philox TAG_STEP output)
+  uint64_t qcap;
+  uint32_t qseg;                  // 1024-position segments per chunk
+};
+
+// Per-epoch step queue of a classifier: which epoch positions each chunk
+// trains on (target or kept negative), in epoch order, with their step draws.
+struct tm_queue_launch {
+  const uint32_t *perm;           // this epoch's order [n]
+  const int64_t *labels;
+  uint32_t n, chunks, clause_chunks, nseg, epoch;
+  uint32_t ek0, ek1, seed_lo, seed_hi;
+  uint64_t skip_threshold;
+  uint64_t *counts;               // [chunks * nseg + 1]
+  uint64_t *offsets;              // [chunks * nseg + 1], exclusive sums; the last is the total
+  uint32_t *qid, *qdraw;
+  uint64_t qcap;
 };
 
 // Work counters of a train launch: TM_NSTATS entries, as documented at
@@ -66,6 +88,12 @@ int tm_launch_setup(uint32_t *counts, uint32_t *actions, uint64_t nclauses, uint
 int tm_launch_perm(uint64_t seed, uint32_t epoch_first, uint32_t epochs, uint32_t n, uint32_t *perm_out,
                    void *scratch, size_t scratch_bytes, hipStream_t st);
 size_t tm_perm_scratch_bytes(uint32_t n);
+// Queue: count + exclusive scan (offsets[chunks * nseg] = total entries), then
+// the write pass into qid / qdraw (capacity qcap >= total).
+uint32_t tm_queue_segments(uint32_t n);
+size_t tm_queue_scratch_bytes(uint64_t items);
+int tm_launch_queue_count(const tm_queue_launch &Q, void *scratch, size_t scratch_bytes, hipStream_t st);
+int tm_launch_queue_write(const tm_queue_launch &Q, hipStream_t st);
 int tm_launch_prepare_predict(const uint32_t *actions, uint32_t *actT, uint32_t *lit, uint32_t clauses_total,
                               uint32_t wd, uint32_t last_eval, hipStream_t st);
 int tm_launch_predict(const tm_predict_launch &L, hipStream_t st);

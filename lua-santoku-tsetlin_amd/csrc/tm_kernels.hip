@@ -159,7 +159,7 @@ __device__ __forceinline__ void load_words(const uint32_t *row, uint32_t wbase, 
 // ---------------------------------------------------------------------------
 // training
 
-template <int WPL, int G, int M, bool IB1>
+template <int WPL, int G, int M, bool IB1, bool QUEUE>
 __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
 {
   constexpr int TW = G * WPL;                  // words per clause tile
@@ -233,38 +233,74 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
     const uint32_t ek0 = P.ekeys[4 * er + 0], ek1 = P.ekeys[4 * er + 1];
     const uint32_t ek2 = P.ekeys[4 * er + 2], ek3 = P.ekeys[4 * er + 3];
 
-    // Sample positions are scanned in blocks of 64, one per lane: lane l holds
-    // the sample id, its target flag and its four step-draw words.  `want`
-    // marks the samples this chunk processes (negative sampling applied).
+    // Samples come in blocks of 64, one per lane: lane l holds the sample id,
+    // its target flag and its step-draw words.  `want` marks the lanes this
+    // chunk trains on.  Without a queue the chunk scans the epoch order itself
+    // (negative sampling applied here); with one, a block is the next 64 queue
+    // entries, loaded one block ahead.
     uint32_t b0 = 0, bs = 0;
     uint64_t want = 0, tmask = 0;
     tm_u4 bd = { 0, 0, 0, 0 };
     uint32_t bsel = 0;                 // full-wave group: this clause's draw word of bd
-    auto scan = [&]() -> bool {       // advance to the next block with work
-      while (b0 < L.n) {
-        const uint32_t pos = b0 + lane;
-        const bool valid = pos < L.n;
-        const uint32_t s = valid ? perm[pos] : 0u;
-        bool tgt, w;
-        if (L.encoder) {
-          tgt = valid && ((L.codes[(size_t) s * L.class_chunks + (cls >> 3)] >> (cls & 7)) & 1);
-          w = valid;
+    uint64_t qb = 0, qe = 0;           // queue entries not yet loaded
+    uint32_t qi_n = 0, qd_n = 0;       // prefetched block
+    tm_u4 qd4_n = { 0, 0, 0, 0 };
+    auto qload = [&]() {
+      const uint64_t idx = qb + lane;
+      if (idx < qe) {
+        qi_n = L.qid[idx];
+        if constexpr (G == 64) {
+          qd_n = L.qdraw[(uint64_t)(j >> 1) * L.qcap + idx];
         } else {
-          const uint32_t sc = valid ? (uint32_t) L.labels[s] : 0xFFFFFFFFu;
-          tgt = valid && sc == cls;
-          w = valid && (tgt || (uint64_t) tm_skip_draw(ek0, ek1, s, ch) >= P.skip_threshold);
-        }
-        b0 += 64;
-        want = __ballot(w);
-        if (want) {
-          tmask = __ballot(tgt);
-          bs = s;
-          bd = tm_philox(s, ch, epoch, TM_TAG_STEP, P.seed_lo, P.seed_hi);
-          if constexpr (G == 64) bsel = (j >> 1) == 0 ? bd.x : (j >> 1) == 1 ? bd.y : (j >> 1) == 2 ? bd.z : bd.w;
-          return true;
+          qd4_n.x = L.qdraw[idx];
+          qd4_n.y = L.qdraw[L.qcap + idx];
+          qd4_n.z = L.qdraw[2 * L.qcap + idx];
+          qd4_n.w = L.qdraw[3 * L.qcap + idx];
         }
       }
-      return false;
+    };
+    if constexpr (QUEUE) {
+      qb = L.qoff[(size_t) ch * L.qseg];
+      qe = L.qoff[(size_t)(ch + 1) * L.qseg];
+      qload();
+    }
+    auto scan = [&]() -> bool {       // advance to the next block with work
+      if constexpr (QUEUE) {
+        if (qb >= qe) return false;
+        const uint64_t left = qe - qb;
+        want = left >= 64 ? ~0ull : ((1ull << left) - 1);
+        bs = qi_n;
+        if constexpr (G == 64) bsel = qd_n;
+        else bd = qd4_n;
+        qb += 64;
+        qload();
+        return true;
+      } else {
+        while (b0 < L.n) {
+          const uint32_t pos = b0 + lane;
+          const bool valid = pos < L.n;
+          const uint32_t s = valid ? perm[pos] : 0u;
+          bool tgt, w;
+          if (L.encoder) {
+            tgt = valid && ((L.codes[(size_t) s * L.class_chunks + (cls >> 3)] >> (cls & 7)) & 1);
+            w = valid;
+          } else {
+            const uint32_t sc = valid ? (uint32_t) L.labels[s] : 0xFFFFFFFFu;
+            tgt = valid && sc == cls;
+            w = valid && (tgt || (uint64_t) tm_skip_draw(ek0, ek1, s, ch) >= P.skip_threshold);
+          }
+          b0 += 64;
+          want = __ballot(w);
+          if (want) {
+            tmask = __ballot(tgt);
+            bs = s;
+            bd = tm_philox(s, ch, epoch, TM_TAG_STEP, P.seed_lo, P.seed_hi);
+            if constexpr (G == 64) bsel = (j >> 1) == 0 ? bd.x : (j >> 1) == 1 ? bd.y : (j >> 1) == 2 ? bd.z : bd.w;
+            return true;
+          }
+        }
+        return false;
+      }
     };
     // take the next wanted sample of the current block (want != 0)
     uint32_t nq_s = 0, nq_d = 0;
@@ -272,8 +308,14 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
     auto take = [&]() {
       const int q = __builtin_ctzll(want);
       want &= want - 1;
-      nq_s = __builtin_amdgcn_readlane(bs, q);
-      nq_t = (tmask >> q) & 1;
+      if constexpr (QUEUE) {
+        const uint32_t v = __builtin_amdgcn_readlane(bs, q);
+        nq_s = v & 0x7FFFFFFFu;
+        nq_t = v >> 31;
+      } else {
+        nq_s = __builtin_amdgcn_readlane(bs, q);
+        nq_t = (tmask >> q) & 1;
+      }
       uint32_t dsel;
       if constexpr (G == 64) {
         dsel = __builtin_amdgcn_readlane(bsel, q);
@@ -375,10 +417,8 @@ __global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
               }
               __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
 #pragma unroll
-              for (int k = 0; k < WPL; k++) {
-                mk[k] = hits[j][wbase + k] & any_one<M, WPL, M>(st, k);
-                hits[j][wbase + k] = 0;
-              }
+              for (int k = 0; k < WPL; k++)     // read and clear in one exchange
+                mk[k] = atomicExch(&hits[j][wbase + k], 0u) & any_one<M, WPL, M>(st, k);
               __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
               sgn = 0u;
             } else if (kind == 1 && grow && phase == 0) {
@@ -654,15 +694,91 @@ __global__ void epoch_advance_kernel(uint32_t *epoch)
   if (threadIdx.x == 0 && blockIdx.x == 0) *epoch += 1;
 }
 
+// ---------------------------------------------------------------------------
+// step queue of a classifier epoch
+
+constexpr uint32_t kQueueSeg = 1024;   // epoch positions per (chunk, segment) wave
+
+// Whether chunk ch trains on epoch position pos: the sample is of the chunk's
+// class, or a negative that the skip draw keeps (the in-kernel scan's test).
+__device__ __forceinline__ bool queue_wanted(const tm_queue_launch &Q, uint32_t pos, uint32_t ch, uint32_t cls,
+                                             uint32_t &s, bool &tgt)
+{
+  const bool valid = pos < Q.n;
+  s = valid ? Q.perm[pos] : 0u;
+  const uint32_t sc = valid ? (uint32_t) Q.labels[s] : 0xFFFFFFFFu;
+  tgt = valid && sc == cls;
+  return valid && (tgt || (uint64_t) tm_skip_draw(Q.ek0, Q.ek1, s, ch) >= Q.skip_threshold);
+}
+
+// grid (chunks, ceil(nseg / 4)), 4 waves per block, one (chunk, segment) each
+__global__ __launch_bounds__(256) void queue_count_kernel(tm_queue_launch Q)
+{
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t seg = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const uint32_t ch = blockIdx.x;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) Q.counts[(size_t) Q.chunks * Q.nseg] = 0;
+  if (seg >= Q.nseg) return;
+  const uint32_t cls = ch / Q.clause_chunks;
+  const uint32_t p0 = seg * kQueueSeg;
+  const uint32_t p1 = (uint32_t) min((uint64_t) Q.n, (uint64_t) p0 + kQueueSeg);
+  uint64_t cnt = 0;
+  for (uint32_t b = p0; b < p1; b += 64) {
+    uint32_t s;
+    bool tgt;
+    const bool w = queue_wanted(Q, b + lane, ch, cls, s, tgt);
+    cnt += __popcll(__ballot(w));
+  }
+  if (lane == 0) Q.counts[(size_t) ch * Q.nseg + seg] = cnt;
+}
+
+// Same grid: each wave writes its segment's entries from its offset on,
+// in epoch order, with the step draws of tm_philox(sample, chunk, epoch).
+__global__ __launch_bounds__(256) void queue_write_kernel(tm_queue_launch Q)
+{
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t seg = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const uint32_t ch = blockIdx.x;
+  if (seg >= Q.nseg) return;
+  const uint32_t cls = ch / Q.clause_chunks;
+  const uint32_t p0 = seg * kQueueSeg;
+  const uint32_t p1 = (uint32_t) min((uint64_t) Q.n, (uint64_t) p0 + kQueueSeg);
+  uint64_t base = Q.offsets[(size_t) ch * Q.nseg + seg];
+  const uint64_t below = (1ull << lane) - 1;
+  for (uint32_t b = p0; b < p1; b += 64) {
+    uint32_t s;
+    bool tgt;
+    const bool w = queue_wanted(Q, b + lane, ch, cls, s, tgt);
+    const uint64_t mask = __ballot(w);
+    if (w) {
+      const uint64_t idx = base + __popcll(mask & below);
+      Q.qid[idx] = s | (tgt ? 0x80000000u : 0u);
+      const tm_u4 d = tm_philox(s, ch, Q.epoch, TM_TAG_STEP, Q.seed_lo, Q.seed_hi);
+      Q.qdraw[idx] = d.x;
+      Q.qdraw[Q.qcap + idx] = d.y;
+      Q.qdraw[2 * Q.qcap + idx] = d.z;
+      Q.qdraw[3 * Q.qcap + idx] = d.w;
+    }
+    base += __popcll(mask);
+  }
+}
+
 template <int WPL, int G>
 int launch_train_shape(const tm_train_launch &L, hipStream_t st)
 {
   if (L.state_bits != 8 || L.row_stride % WPL != 0 || L.row_stride < L.wd || (uint64_t) G * WPL < L.wd)
     return (int) hipErrorInvalidValue;
-  if (L.all_ib1)
-    hipLaunchKernelGGL((train_kernel<WPL, G, 7, true>), dim3(L.total_blocks), dim3(8 * G), 0, st, L);
-  else
-    hipLaunchKernelGGL((train_kernel<WPL, G, 7, false>), dim3(L.total_blocks), dim3(8 * G), 0, st, L);
+  const dim3 grid(L.total_blocks), block(8 * G);
+  if (L.qid) {
+    // the queue serves one classifier machine for one epoch
+    if (L.trials || L.block_map || L.epoch_dev || L.encoder || L.epochs != 1 || !L.qoff || !L.qdraw)
+      return (int) hipErrorInvalidValue;
+    if (L.all_ib1) hipLaunchKernelGGL((train_kernel<WPL, G, 7, true, true>), grid, block, 0, st, L);
+    else hipLaunchKernelGGL((train_kernel<WPL, G, 7, false, true>), grid, block, 0, st, L);
+  } else {
+    if (L.all_ib1) hipLaunchKernelGGL((train_kernel<WPL, G, 7, true, false>), grid, block, 0, st, L);
+    else hipLaunchKernelGGL((train_kernel<WPL, G, 7, false, false>), grid, block, 0, st, L);
+  }
   return (int) hipGetLastError();
 }
 
@@ -767,6 +883,35 @@ int tm_launch_perm(uint64_t seed, uint32_t epoch_first, uint32_t epochs, uint32_
                                                         perm_out + (size_t) e * n, (int) n, 0, 32, st);
     if (err != hipSuccess) return (int) err;
   }
+  return (int) hipGetLastError();
+}
+
+uint32_t tm_queue_segments(uint32_t n) { return (uint32_t)(((uint64_t) n + kQueueSeg - 1) / kQueueSeg); }
+
+size_t tm_queue_scratch_bytes(uint64_t items)
+{
+  size_t temp = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, temp, (uint64_t *) nullptr, (uint64_t *) nullptr, (int) items);
+  return std::max<size_t>(temp, 256);
+}
+
+int tm_launch_queue_count(const tm_queue_launch &Q, void *scratch, size_t scratch_bytes, hipStream_t st)
+{
+  const uint64_t items = (uint64_t) Q.chunks * Q.nseg + 1;
+  if (Q.n == 0 || Q.nseg != tm_queue_segments(Q.n) || items > 0x7FFFFFFFull || (Q.nseg + 3) / 4 > 65535)
+    return (int) hipErrorInvalidValue;
+  hipLaunchKernelGGL(queue_count_kernel, dim3(Q.chunks, (Q.nseg + 3) / 4), dim3(256), 0, st, Q);
+  size_t tb = scratch_bytes;
+  hipError_t err = hipcub::DeviceScan::ExclusiveSum(scratch, tb, Q.counts, Q.offsets, (int) items, st);
+  if (err != hipSuccess) return (int) err;
+  return (int) hipGetLastError();
+}
+
+int tm_launch_queue_write(const tm_queue_launch &Q, hipStream_t st)
+{
+  if (!Q.qid || !Q.qdraw || Q.nseg != tm_queue_segments(Q.n) || (Q.nseg + 3) / 4 > 65535)
+    return (int) hipErrorInvalidValue;
+  hipLaunchKernelGGL(queue_write_kernel, dim3(Q.chunks, (Q.nseg + 3) / 4), dim3(256), 0, st, Q);
   return (int) hipGetLastError();
 }
 

@@ -191,6 +191,12 @@ class Machine:
         _check(self._lib.tm_last_timing(self._handle(), ctypes.byref(t), ctypes.byref(p)))
         return t.value, p.value
 
+    def queue_timing(self):
+        """Milliseconds the last train call spent building step queues."""
+        q = ctypes.c_double(0)
+        _check(self._lib.tm_last_queue_timing(self._handle(), ctypes.byref(q)))
+        return q.value
+
     def train_stats(self):
         """Work counters of the last train call (see tm_last_train_stats)."""
         out = (ctypes.c_uint64 * 6)()

@@ -24,6 +24,7 @@ EXPORTS = [
     "tm_predict_encoder", "tm_predict_classifier_dev", "tm_predict_encoder_dev", "tm_persist",
     "tm_persist_file", "tm_load", "tm_load_file", "tm_checkpoint", "tm_restore", "tm_reconfigure",
     "tm_type", "tm_destroy", "tm_get_info", "tm_last_error", "tm_last_timing", "tm_last_train_stats",
+    "tm_last_queue_timing",
     "tm_batch_create", "tm_batch_machine", "tm_batch_train_classifier", "tm_batch_train_classifier_dev",
     "tm_batch_accuracy_dev", "tm_batch_tuning", "tm_batch_timing", "tm_batch_destroy",
 ]
@@ -101,6 +102,7 @@ def load_library(path=LIB_PATH):
         "tm_last_error": (ctypes.c_char_p, []),
         "tm_last_timing": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "tm_last_train_stats": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "tm_last_queue_timing": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double)]),
         "tm_batch_create": (ctypes.c_int, [opt_p, u32, m_pp]),
         "tm_batch_machine": (ctypes.c_int, [vp, u32, m_pp]),
         "tm_batch_train_classifier": (ctypes.c_int, [vp, u32, u8p, i64p, u32, BATCH_EPOCH_FN, vp]),

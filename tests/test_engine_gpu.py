@@ -151,6 +151,41 @@ def test_negative_sampling_variants():
         assert np.array_equal(eng.checkpoint(), orc.actions), neg
 
 
+QUEUE_SHAPES = [
+    # features, classes, clauses, negative, n -- tiles 8x1, 16x2, 64x1, 64x4, 64x16
+    (5, 3, 16, 0.5, 300),
+    (200, 4, 24, 0.0, 500),
+    (1024, 3, 16, 1.0, 400),
+    (4096, 5, 8, 0.1, 700),
+    (16384, 2, 8, 0.3, 90),
+]
+
+
+@pytest.mark.parametrize("features,classes,clauses,neg,n", QUEUE_SHAPES)
+def test_step_queue_matches_scan(features, classes, clauses, neg, n, monkeypatch):
+    """A classifier trains from per-epoch step queues; the in-kernel scan of
+    the epoch order (TSETLIN_AMD_QUEUE=0) must give the same machine and the
+    same work counters, and both the oracle's replay."""
+    rng = np.random.default_rng(features + n)
+    rows = rows_for(features, n, rng, 0.3)
+    labels = rng.integers(0, classes, n).astype(np.int64)
+    labels[::13] = -1
+    o = dict(features=features, classes=classes, clauses=clauses, negative=neg, seed=21, target=6,
+             clause_tolerance=5, clause_maximum=7, specificity=max(2.0, features / 16.0))
+    got = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("TSETLIN_AMD_QUEUE", mode)
+        eng, orc = pair(**o)
+        eng.train(samples=n, problems=rows, solutions=labels, iterations=3)
+        got[mode] = (eng.checkpoint(), eng.train_stats(), eng.queue_timing())
+    assert np.array_equal(got["1"][0], got["0"][0])
+    assert got["1"][1] == got["0"][1]
+    assert got["1"][2] > 0 and got["0"][2] == 0
+    orc.train_replay(rows, labels=labels, epochs=3)
+    assert np.array_equal(got["1"][0], orc.actions)
+    assert got["1"][1]["steps"] == int(orc.stats[0])
+
+
 def test_each_callback_epochs_and_stop():
     rng = np.random.default_rng(6)
     rows = rows_for(64, 200, rng)
