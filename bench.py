@@ -10,8 +10,12 @@ samples/s; predict samples/s over the 20k test rows is reported beside it.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 
 The roofline object prices the train kernel against the integer VALU peak
-(256 CU x 64 lanes x 2.4 GHz): its algorithmic operations are computed from
-the kernel's own work counters (chunk-steps evaluated and feedback events).
+of gfx950: 256 CUs x 4 SIMDs x 32 lanes x 2.4 GHz = 78.6 T lane-ops/s for
+full-rate 32-bit ops (xor, three-input logic, add issue a wave64 op every two
+cycles; popcount every four, so it counts as two).  Its algorithmic operations
+come from the kernel's own work counters (chunk-steps evaluated and feedback
+events); `traffic` is the kernel's HBM bytes per launch from the committed
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary.
 cpu_baseline times the oracle's reference-shaped CPU path (OpenMP over
 8-clause chunks, -O3 -march=native) on a bounded slice of the same workload.
 """
@@ -28,7 +32,7 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 METRIC = "train & predict samples/s @2048 clauses/class; search time-to-acc, surrogate vs plain"
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # int32 lane-ops/s
+VALU_PEAK_TOPS = 256 * 4 * 32 * 2.4e9 / 1e12     # full-rate int32 lane-ops/s
 HBM_PEAK_GBS = 8000.0
 
 MACHINE = dict(classes=20, features=4096, clauses=2048, clause_tolerance=64, clause_maximum=64,
@@ -40,14 +44,35 @@ def log(*a):
 
 
 def algorithmic_ops(stats, wd, m=7):
-    """Integer lane-ops of the train kernel from its work counters.
-    eval: 8 clauses x wd words x (AND-NOT, popcount-accumulate) per chunk-step;
-    a full-row feedback pass: (3(m+1)+3) ops per word; a random-literal
-    decrement: 2(m+1) ops."""
-    eval_ops = stats["steps"] * 16 * wd
+    """Full-rate integer lane-ops of the train kernel from its work counters
+    (a half-rate popcount counts twice).
+    eval: 8 clauses x wd words x (AND-NOT with the word mask, popcount-
+    accumulate) = 3 per word per clause and chunk-step;
+    a full-row feedback pass: the literal mask plus a carry/borrow ripple of
+    two ops per plane = 2(m+1)+1 per word;
+    a literal recount where the action plane can change (Type II, growth,
+    Type I on a false clause): 3 per word;
+    a random-literal decrement: 2(m+1) ops on its word."""
+    eval_ops = stats["steps"] * 8 * 3 * wd
     passes = stats["type1_true"] + stats["type1_grow"] + stats["type2_true"]
-    fb_ops = passes * (3 * (m + 1) + 3) * wd + stats["draws"] * 2 * (m + 1)
+    recounts = stats["type2_true"] + stats["type1_grow"] + stats["type1_false"]
+    fb_ops = passes * (2 * (m + 1) + 1) * wd + recounts * 3 * wd + stats["draws"] * 2 * (m + 1)
     return eval_ops + fb_ops, eval_ops, fb_ops
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "train_kernel_classifier_200k_4096f_20c_2048cl.md")
+
+
+def profiled_traffic():
+    """HBM bytes per train launch (one epoch) from the committed profile
+    summary, or None."""
+    import re
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", f.read(), re.M)
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def cpu_baseline(data, threads, budget_s=15.0):
@@ -297,11 +322,12 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     train_kernel_ms, _ = eng.timing()
+    queue_ms = eng.queue_timing()
     stats = eng.train_stats()
     ops, eval_ops, fb_ops = algorithmic_ops(stats, wd)
     train_sps = args.steps * n / wall
-    log("train: %d epochs in %.3f s (kernel %.1f ms) -> %.0f samples/s; stats %s" % (
-        args.steps, wall, train_kernel_ms, train_sps, stats))
+    log("train: %d epochs in %.3f s (kernel %.1f ms, step queues %.1f ms) -> %.0f samples/s; stats %s" % (
+        args.steps, wall, train_kernel_ms, queue_ms, train_sps, stats))
 
     pred = eng.predict(test_rows, ntest)
     torch.cuda.synchronize()
@@ -316,7 +342,8 @@ def main():
     log("predict: %d rows in %.2f ms (kernel %.2f ms); test accuracy %.4f" % (ntest, pwall * 1e3, predict_kernel_ms, acc))
 
     achieved = ops / (train_kernel_ms / 1e3) / 1e12
-    hbm_bytes = args.steps * (n * data["train_rows"].shape[1] + n * 8) + 2 * eng.info()["action_chunks"] * 8
+    # unique bytes per epoch launch: every row and label once, the state planes in and out
+    hbm_bytes = args.steps * (n * data["train_rows"].shape[1] + n * 8 + 2 * eng.info()["action_chunks"] * 8)
     line = {
         "metric": METRIC,
         "value": round(train_sps, 1),
@@ -336,6 +363,7 @@ def main():
         "predict_samples_per_s": round(ntest / pwall, 1),
         "predict_kernel_ms": round(predict_kernel_ms, 3),
         "train_kernel_ms_per_epoch": round(train_kernel_ms / args.steps, 3),
+        "queue_ms_per_epoch": round(queue_ms / args.steps, 3),
         "test_accuracy_after_steps": round(acc, 4),
         "roofline": {
             "bound": "valu",
@@ -343,11 +371,12 @@ def main():
             "peak": round(VALU_PEAK_TOPS, 2),
             "unit": "Tops/s (int32 lane-ops)",
             "frac": round(achieved / VALU_PEAK_TOPS, 4),
-            "traffic": None,
-            "kernel": "train_kernel<4,64,7>",
-            "ops_per_launch": ops,
-            "eval_ops": eval_ops,
-            "feedback_ops": fb_ops,
+            "traffic": profiled_traffic(),
+            "kernel": "train_kernel<4,64,7,ib1,queue> (one launch per epoch)",
+            "launches": args.steps,
+            "ops_per_launch": ops // args.steps,
+            "eval_ops_per_launch": eval_ops // args.steps,
+            "feedback_ops_per_launch": fb_ops // args.steps,
             "hbm_algorithmic_gbs": round(hbm_bytes / (train_kernel_ms / 1e3) / 1e9, 2),
             "hbm_peak_gbs": HBM_PEAK_GBS,
         },

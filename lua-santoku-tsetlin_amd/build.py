@@ -1,8 +1,11 @@
 """Build libtsetlin_amd.so in place (hipcc, gfx950 only).
 
-    python lua-santoku-tsetlin_amd/build.py
+    python lua-santoku-tsetlin_amd/build.py            # the library
+    python lua-santoku-tsetlin_amd/build.py --probe    # + probe_valu_issue
 
 Objects go to lua-santoku-tsetlin_amd/_build/, the library next to this file.
+probe_valu_issue (next to this file too) measures the integer VALU issue rate
+that prices the train kernel's roofline.
 """
 import os
 import subprocess
@@ -50,6 +53,21 @@ def build(verbose=False):
     return OUT
 
 
+def build_probe(verbose=False):
+    out = os.path.join(PKG, "probe_valu_issue")
+    cmd = [HIPCC, "-O3", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Wno-unused-command-line-argument", "-Wno-unused-result", os.path.join(CSRC, "probe_valu_issue.hip"),
+           "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for the probe:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    if "--probe" in sys.argv[1:]:
+        build_probe(verbose=True)
     sys.exit(0)
