@@ -122,6 +122,161 @@ def cpu_baseline(data, threads, budget_s=15.0):
                 predict_s=tp)
 
 
+# workload 3: plain encoder, 8192 visible Bernoulli(0.1) bits -> 256 hidden
+# bits (sign of a fixed Gaussian projection of x - 0.1, 3% flips)
+ENCODER = dict(hidden=256, visible=8192, clauses=1024, clause_tolerance=64, clause_maximum=64, specificity=20.0,
+               target=128, seed=2025)
+
+
+def encoder_rows_gpu(n, seed, dev, block=25_000):
+    """Workload-3 data generated on the device: flip-interleaved rows
+    (n x 2048 bytes) and packed codes (n x 32 bytes)."""
+    import torch
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    w = torch.randn((8192, 256), generator=g, device=dev)
+    rows = torch.empty((n, 2048), dtype=torch.uint8, device=dev)
+    codes = torch.empty((n, 32), dtype=torch.uint8, device=dev)
+    wt = (2 ** torch.arange(8, device=dev, dtype=torch.int32)).view(1, 1, 8)
+    for s0 in range(0, n, block):
+        b = min(block, n - s0)
+        x = torch.rand((b, 8192), generator=g, device=dev) < 0.1
+        h = ((x.float() - 0.1) @ w) > 0
+        h ^= torch.rand(h.shape, generator=g, device=dev) < 0.03
+        lits = torch.cat([x, ~x], dim=1).view(b, 2048, 8).to(torch.int32)
+        rows[s0:s0 + b] = (lits * wt).sum(2).to(torch.uint8)
+        codes[s0:s0 + b] = (h.view(b, 32, 8).to(torch.int32) * wt).sum(2).to(torch.uint8)
+    return rows, codes
+
+
+def cpu_baseline_encoder(rows, codes, threads, budget_s=15.0):
+    """Reference-shaped CPU encoder training on a bounded slice (one epoch over
+    the first `limit` samples of each thread's shuffle, all 32,768 chunks)
+    and predict over a bounded slice."""
+    import numpy as np
+    from oracle import oracle as O
+    try:
+        so = O.build(native=True)
+    except Exception as ex:
+        log("native oracle build failed (%s); using the portable build" % ex)
+        so = O.build(native=False)
+    L = O.lib(so)
+    p = O.params("encoder", **{("classes" if k == "hidden" else "features" if k == "visible" else k): v
+                               for k, v in ENCODER.items()})
+
+    def run(limit):
+        m = O.Machine(p)
+        t0 = time.perf_counter()
+        m.train_refshape(rows, codes=codes, epochs=1, threads=threads, sample_limit=limit, L=L)
+        return time.perf_counter() - t0, m
+
+    run(1)
+    limit = 2
+    t, m = run(limit)
+    while t < budget_s / 4 and limit < rows.shape[0]:
+        limit = min(rows.shape[0], int(limit * min(8.0, max(2.0, budget_s / 2 / max(t, 1e-3)))))
+        t, m = run(limit)
+    npred = 4
+    while True:
+        out = np.zeros((npred, 32), np.uint8)
+        t0 = time.perf_counter()
+        L.orc_predict_encoder_threads(O.ctypes.byref(p), O.ptr(m.actions), O.ptr(rows[:npred]), npred,
+                                      O.ptr(out), threads)
+        tp = time.perf_counter() - t0
+        if tp >= 2.0 or npred >= rows.shape[0]:
+            break
+        npred = min(npred * 4, rows.shape[0])
+    return dict(train_sps=limit / t, train_sample=limit, train_s=t, predict_sps=npred / tp, predict_sample=npred,
+                predict_s=tp)
+
+
+def encoder_bench(args):
+    """Workload 3 at its full machine shape: one epoch over `--enc-train`
+    samples (per-sample cost does not depend on n) and predict over
+    `--enc-predict` samples."""
+    import numpy as np
+    import torch
+    import santoku.tsetlin as tm
+    dev = torch.device("cuda:0")
+    n_tr, n_pr = args.enc_train, args.enc_predict
+    t0 = time.perf_counter()
+    rows, codes = encoder_rows_gpu(max(n_tr + 8192, n_pr), 17, dev)
+    torch.cuda.synchronize()
+    log("encoder data: %d rows of 2048 bytes + 32-byte codes (%.1fs)" % (rows.shape[0], time.perf_counter() - t0))
+    eng = tm.encoder(dict(ENCODER, reusable=True))
+    wd = (eng.info()["input_chunks"] + 3) // 4
+    eng.train(samples=256, sentences=rows, codes=codes, iterations=1)     # module load, warm caches
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.train(samples=n_tr, sentences=rows, codes=codes, iterations=1)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    train_ms, _ = eng.timing()
+    stats = eng.train_stats()
+    ops, eval_ops, fb_ops = algorithmic_ops(stats, wd)
+    log("encoder train: %d samples x 1 epoch in %.2f s (kernel %.1f ms); stats %s" % (n_tr, wall, train_ms, stats))
+    held = rows[n_tr:n_tr + 8192]
+    pred = eng.predict(held.contiguous(), 8192).cpu().numpy()
+    want = codes[n_tr:n_tr + 8192].cpu().numpy()
+    bit_err = float(np.unpackbits(pred ^ want).mean())
+    eng.predict(rows[:4096].contiguous(), 4096)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.predict(rows[:n_pr], n_pr)
+    torch.cuda.synchronize()
+    pwall = time.perf_counter() - t0
+    _, predict_ms = eng.timing()
+    log("encoder predict: %d rows in %.2f s (kernel %.1f ms); held-out bit error %.4f" % (
+        n_pr, pwall, predict_ms, bit_err))
+    pred_ops = n_pr * 256 * 1024 * wd * 3
+    line = {
+        "metric": "encoder train & predict samples/s (workload 3)",
+        "value": round(n_tr / wall, 2),
+        "unit": "train samples/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "u32",
+        "data": "synthetic workload-3 data generated on the device",
+        "config": {"workload": "plain encoder, 8192 Bernoulli(0.1) visible bits -> 256 hidden bits (Gaussian "
+                               "projection sign, 3%% flips), 1024 clauses per bit, target 128, specificity 20; "
+                               "one epoch over %d samples" % n_tr, "samples": n_tr},
+        "train_kernel_ms": round(train_ms, 2),
+        "predict_samples_per_s": round(n_pr / pwall, 1),
+        "predict_kernel_ms": round(predict_ms, 2),
+        "held_out_bit_error_after_epoch": round(bit_err, 4),
+        "roofline": {"bound": "valu", "kernel": "train_kernel<8,64,7,ib1,scan>",
+                     "achieved": round(ops / (train_ms / 1e3) / 1e12, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                     "unit": "Tops/s (int32 lane-ops)", "frac": round(ops / (train_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 4),
+                     "eval_ops": eval_ops, "feedback_ops": fb_ops,
+                     "predict_frac": round(pred_ops / (predict_ms / 1e3) / 1e12 / VALU_PEAK_TOPS, 4)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        import subprocess
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            np.save(os.path.join(td, "rows.npy"), rows[:4096].cpu().numpy())
+            np.save(os.path.join(td, "codes.npy"), codes[:4096].cpu().numpy())
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-only", "--encoder", "--data", td],
+                               capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("cpu baseline failed (%d): %s" % (r.returncode, r.stderr[-2000:]))
+        cb = json.loads(r.stdout.strip().splitlines()[-1])
+        log("cpu baseline: %d threads, %.3f train samples/s, %.2f predict samples/s" % (
+            cb["threads"], cb["train_sps"], cb["predict_sps"]))
+        line["cpu_baseline"] = {
+            "value": round(cb["train_sps"], 4), "unit": "train samples/s", "cores": cb["threads"], "kind": "port",
+            "sample": "oracle reference-shaped encoder training (OpenMP static over 32,768 chunks), one epoch "
+                      "restricted to the first %d samples of each thread's shuffle, %.1f s" % (
+                          cb["train_sample"], cb["train_s"]),
+            "predict_value": round(cb["predict_sps"], 3),
+            "predict_sample": "%d rows, %.2f s" % (cb["predict_sample"], cb["predict_s"]),
+            "speedup_train": round((n_tr / wall) / cb["train_sps"], 1),
+            "speedup_predict": round((n_pr / pwall) / cb["predict_sps"], 1),
+        }
+    print(json.dumps(line), flush=True)
+
+
 SEARCH_SPACE = {
     # workload 4/5: clauses log-uniform in 64..4096 per class, specificity
     # uniform in 2..40, target clauses/8 or clauses/4; tolerance and maximum
@@ -264,9 +419,22 @@ def main():
     ap.add_argument("--cpu-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--search", action="store_true", help="batched trials and surrogate search (workloads 4, 5)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--encoder", action="store_true", help="workload 3: the 256-bit encoder at full machine shape")
+    ap.add_argument("--enc-train", type=int, default=16_384)
+    ap.add_argument("--enc-predict", type=int, default=500_000)
+    ap.add_argument("--data", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="plain 1 GiB device copies, to scale the FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.cpu_only and args.encoder:
+        import numpy as np
+        rows = np.load(os.path.join(args.data, "rows.npy"))
+        codes = np.load(os.path.join(args.data, "codes.npy"))
+        threads = len(os.sched_getaffinity(0))
+        cb = cpu_baseline_encoder(rows, codes, threads)
+        cb["threads"] = threads
+        print(json.dumps(cb), flush=True)
+        return
     if args.cpu_only:
         # child process of the CPU leg: no torch / HIP in this process, so the
         # oracle's OpenMP runtime is the only one loaded
@@ -283,6 +451,9 @@ def main():
         sys.exit(2)
     if args.search:
         search_bench(args)
+        return
+    if args.encoder:
+        encoder_bench(args)
         return
     if args.copy_calibration:
         import torch

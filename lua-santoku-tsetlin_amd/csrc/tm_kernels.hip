@@ -650,28 +650,30 @@ __global__ void argmax_kernel(const int32_t *sums, uint32_t n, uint32_t classes,
 __global__ void encode_kernel(const int32_t *sums, uint32_t n, uint32_t classes, uint8_t *out)
 {
   const uint32_t cc = (classes + 7) / 8;
-  uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (uint64_t) n * cc) return;
-  const uint32_t s = (uint32_t)(i / cc), b = (uint32_t)(i % cc);
-  uint8_t v = 0;
-  for (uint32_t t = 0; t < 8; t++) {
-    uint32_t k = b * 8 + t;
-    if (k < classes && sums[(size_t) s * classes + k] > 0) v |= (uint8_t)(1u << t);
+  const uint64_t total = (uint64_t) n * cc, stride = (uint64_t) gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    const uint32_t s = (uint32_t)(i / cc), b = (uint32_t)(i % cc);
+    uint8_t v = 0;
+    for (uint32_t t = 0; t < 8; t++) {
+      uint32_t k = b * 8 + t;
+      if (k < classes && sums[(size_t) s * classes + k] > 0) v |= (uint8_t)(1u << t);
+    }
+    out[i] = v;
   }
-  out[i] = v;
 }
 
 __global__ void repack_kernel(const uint8_t *src, uint32_t n, uint32_t bytes, uint32_t wd, uint32_t *dst)
 {
-  uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (uint64_t) n * wd) return;
-  const uint64_t s = i / wd, w = i % wd;
-  uint32_t v = 0;
-  for (uint32_t b = 0; b < 4; b++) {
-    uint64_t o = w * 4 + b;
-    if (o < bytes) v |= (uint32_t) src[s * bytes + o] << (8 * b);
+  const uint64_t total = (uint64_t) n * wd, stride = (uint64_t) gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    const uint64_t s = i / wd, w = i % wd;
+    uint32_t v = 0;
+    for (uint32_t b = 0; b < 4; b++) {
+      uint64_t o = w * 4 + b;
+      if (o < bytes) v |= (uint32_t) src[s * bytes + o] << (8 * b);
+    }
+    dst[i] = v;
   }
-  dst[i] = v;
 }
 
 __global__ void zero_kernel(uint32_t *p, uint64_t nwords)
@@ -768,6 +770,8 @@ int launch_train_shape(const tm_train_launch &L, hipStream_t st)
 {
   if (L.state_bits != 8 || L.row_stride % WPL != 0 || L.row_stride < L.wd || (uint64_t) G * WPL < L.wd)
     return (int) hipErrorInvalidValue;
+  // one workgroup per chunk: the grid must stay below 2^32 work-items
+  if ((uint64_t) L.total_blocks * 8 * G > 0xFFFFFFFFull) return (int) hipErrorInvalidValue;
   const dim3 grid(L.total_blocks), block(8 * G);
   if (L.qid) {
     // the queue serves one classifier machine for one epoch
@@ -927,15 +931,28 @@ int tm_launch_prepare_predict(const uint32_t *actions, uint32_t *actT, uint32_t 
 
 int tm_launch_predict(const tm_predict_launch &L, hipStream_t st)
 {
-  const uint64_t nst = (L.n + PRED_WAVES * PRED_SPW - 1) / (PRED_WAVES * PRED_SPW);
+  const uint64_t spt = PRED_WAVES * PRED_SPW;            // samples per tile
   const uint64_t nct = (L.clauses_total + 63) / 64;
-  if (nst * nct > 0x7FFFFFFFull) return (int) hipErrorInvalidValue;
+  // A dispatch's grid is at most 2^32 - 1 work-items: larger sample counts
+  // go out as several launches over consecutive sample ranges.
+  const uint64_t max_blocks = 0xFFFFFFFFull / (64 * PRED_WAVES);
+  if (nct > max_blocks) return (int) hipErrorInvalidValue;
+  const uint64_t st_per_launch = max_blocks / nct;
   tm_predict_launch P = L;
   // clause tiles per pass: about 2 MiB of include plane
   const uint64_t tile_bytes = 64ull * L.wd * 4;
   P.pass_tiles = (uint32_t) std::max<uint64_t>(1, std::min<uint64_t>(nct, (2ull << 20) / tile_bytes));
-  hipLaunchKernelGGL(predict_kernel, dim3((uint32_t)(nst * nct)), dim3(64 * PRED_WAVES), 0, st, P);
-  return (int) hipGetLastError();
+  for (uint64_t s0 = 0; s0 < L.n; s0 += st_per_launch * spt) {
+    const uint64_t n = std::min<uint64_t>(L.n - s0, st_per_launch * spt);
+    P.rows = L.rows + s0 * L.wd;
+    P.sums = L.sums + s0 * L.classes;
+    P.n = (uint32_t) n;
+    const uint64_t nst = (n + spt - 1) / spt;
+    hipLaunchKernelGGL(predict_kernel, dim3((uint32_t)(nst * nct)), dim3(64 * PRED_WAVES), 0, st, P);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int) e;
+  }
+  return (int) hipSuccess;
 }
 
 int tm_launch_argmax(const int32_t *sums, uint32_t n, uint32_t classes, int64_t *out, hipStream_t st)
@@ -946,15 +963,17 @@ int tm_launch_argmax(const int32_t *sums, uint32_t n, uint32_t classes, int64_t 
 
 int tm_launch_encode(const int32_t *sums, uint32_t n, uint32_t classes, uint8_t *out, hipStream_t st)
 {
-  uint64_t total = (uint64_t) n * ((classes + 7) / 8);
-  hipLaunchKernelGGL(encode_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, sums, n, classes, out);
+  const uint64_t total = (uint64_t) n * ((classes + 7) / 8);
+  const uint32_t blocks = (uint32_t) std::min<uint64_t>((total + 255) / 256, 1u << 16);
+  hipLaunchKernelGGL(encode_kernel, dim3(std::max(blocks, 1u)), dim3(256), 0, st, sums, n, classes, out);
   return (int) hipGetLastError();
 }
 
 int tm_launch_repack(const uint8_t *src, uint32_t n, uint32_t bytes, uint32_t wd, uint32_t *dst, hipStream_t st)
 {
-  uint64_t total = (uint64_t) n * wd;
-  hipLaunchKernelGGL(repack_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, src, n, bytes, wd, dst);
+  const uint64_t total = (uint64_t) n * wd;
+  const uint32_t blocks = (uint32_t) std::min<uint64_t>((total + 255) / 256, 1u << 16);
+  hipLaunchKernelGGL(repack_kernel, dim3(std::max(blocks, 1u)), dim3(256), 0, st, src, n, bytes, wd, dst);
   return (int) hipGetLastError();
 }
 

@@ -88,6 +88,7 @@ def _bind(L):
         "orc_class_sums": (None, [P, _U8P, _U8P, ctypes.c_uint32, _I64P]),
         "orc_predict_classifier": (None, [P, _U8P, _U8P, ctypes.c_uint32, _I64P]),
         "orc_predict_classifier_threads": (None, [P, _U8P, _U8P, ctypes.c_uint32, _I64P, ctypes.c_int]),
+        "orc_predict_encoder_threads": (None, [P, _U8P, _U8P, ctypes.c_uint32, _U8P, ctypes.c_int]),
         "orc_predict_encoder": (None, [P, _U8P, _U8P, ctypes.c_uint32, _U8P]),
         "orc_s_draws": (ctypes.c_uint32, [P]),
         "orc_train_replay": (None, [P, _U8P, _U8P, ctypes.c_uint32, _U8P, _I64P, _U8P,

@@ -731,4 +731,12 @@ void orc_predict_classifier_threads (const orc_params *p, const uint8_t *actions
   omp_set_num_threads(prev);
 }
 
+void orc_predict_encoder_threads (const orc_params *p, const uint8_t *actions, const uint8_t *rows, uint32_t n, uint8_t *out, int nthreads)
+{
+  int prev = omp_get_max_threads();
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  orc_predict_encoder(p, actions, rows, n, out);
+  omp_set_num_threads(prev);
+}
+
 int orc_max_threads (void) { return omp_get_max_threads(); }
