@@ -160,7 +160,7 @@ __device__ __forceinline__ void load_words(const uint32_t *row, uint32_t wbase, 
 // training
 
 template <int WPL, int G, int M, bool IB1, bool QUEUE>
-__global__ __launch_bounds__(8 * G) void train_kernel(tm_train_launch L)
+__global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel(tm_train_launch L)
 {
   constexpr int TW = G * WPL;                  // words per clause tile
   const uint32_t tid = threadIdx.x;
