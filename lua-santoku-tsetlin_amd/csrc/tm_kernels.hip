@@ -245,24 +245,24 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
     uint64_t qb = 0, qe = 0;           // queue entries not yet loaded
     uint32_t qi_n = 0, qd_n = 0;       // prefetched block
     tm_u4 qd4_n = { 0, 0, 0, 0 };
-    auto qload = [&]() {
-      const uint64_t idx = qb + lane;
-      if (idx < qe) {
-        qi_n = L.qid[idx];
-        if constexpr (G == 64) {
-          qd_n = L.qdraw[(uint64_t)(j >> 1) * L.qcap + idx];
-        } else {
-          qd4_n.x = L.qdraw[idx];
-          qd4_n.y = L.qdraw[L.qcap + idx];
-          qd4_n.z = L.qdraw[2 * L.qcap + idx];
-          qd4_n.w = L.qdraw[3 * L.qcap + idx];
-        }
+    auto qload = [&]() {              // called with qb < qe
+      // lanes past the chunk's last entry re-read it: the load stays
+      // unconditional, so it can land in place without a wait
+      const uint64_t idx = min(qb + lane, qe - 1);
+      qi_n = L.qid[idx];
+      if constexpr (G == 64) {
+        qd_n = L.qdraw[(uint64_t)(j >> 1) * L.qcap + idx];
+      } else {
+        qd4_n.x = L.qdraw[idx];
+        qd4_n.y = L.qdraw[L.qcap + idx];
+        qd4_n.z = L.qdraw[2 * L.qcap + idx];
+        qd4_n.w = L.qdraw[3 * L.qcap + idx];
       }
     };
     if constexpr (QUEUE) {
       qb = L.qoff[(size_t) ch * L.qseg];
       qe = L.qoff[(size_t)(ch + 1) * L.qseg];
-      qload();
+      if (qb < qe) qload();
     }
     auto scan = [&]() -> bool {       // advance to the next block with work
       if constexpr (QUEUE) {
@@ -270,10 +270,15 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
         const uint64_t left = qe - qb;
         want = left >= 64 ? ~0ull : ((1ull << left) - 1);
         bs = qi_n;
-        if constexpr (G == 64) bsel = qd_n;
-        else bd = qd4_n;
+        asm volatile("" : "+v"(bs));
+        if constexpr (G == 64) {
+          bsel = qd_n;
+          asm volatile("" : "+v"(bsel));
+        } else {
+          bd = qd4_n;
+        }
         qb += 64;
-        qload();
+        if (qb < qe) qload();
         return true;
       } else {
         while (b0 < L.n) {
@@ -335,9 +340,16 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
     for (;;) {
       const uint32_t sq = nq_s, u16 = nq_d;
       const bool tq = nq_t;
+      // x is defined here, by an empty asm on the copy: without it the
+      // compiler renames x onto the prefetch registers across its loop
+      // rotation and then waits for the next row's load (vmcnt(0)) before
+      // this step's evaluation, exposing the prefetch latency every step
       uint32_t x[WPL];
 #pragma unroll
-      for (int k = 0; k < WPL; k++) x[k] = xn[k];
+      for (int k = 0; k < WPL; k++) {
+        x[k] = xn[k];
+        asm volatile("" : "+v"(x[k]));
+      }
       // find the next sample and start loading its row before this one's work
       const bool more = want ? true : scan();
       if (more) {
