@@ -131,16 +131,12 @@ __device__ __forceinline__ uint32_t included(const uint32_t (&st)[M + 1][WPL], i
   return v;
 }
 
-// Load WPL consecutive words of a literal row (the row stride is a multiple
-// of WPL, so a lane's words are all inside the row or all past its end).
+// Load WPL consecutive words of a literal row from word wbase.  Callers
+// clamp wbase into the row (the row stride is a multiple of WPL) so the load
+// is unconditional; words past the row's end are masked by every use.
 template <int WPL>
-__device__ __forceinline__ void load_words(const uint32_t *row, uint32_t wbase, uint32_t wd, uint32_t (&x)[WPL])
+__device__ __forceinline__ void load_words(const uint32_t *row, uint32_t wbase, uint32_t (&x)[WPL])
 {
-  if (wbase >= wd) {
-#pragma unroll
-    for (int k = 0; k < WPL; k++) x[k] = 0;
-    return;
-  }
   if constexpr (WPL % 4 == 0) {
 #pragma unroll
     for (int k = 0; k < WPL; k += 4) {
@@ -223,6 +219,16 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
 
   const uint32_t T = P.target;
   const uint32_t tol = P.tolerance;
+  // votes = max(0, cap - failed): a clause without literals fails nothing and
+  // votes tol, else min(literals, tol) - failed (capi.c:110-119); cap only
+  // changes with the literal count
+  auto cap_of = [&](uint32_t l) -> int32_t { return (int32_t)(l == 0 ? tol : (l < tol ? l : tol)); };
+  int32_t cap = cap_of(lit);
+  const int32_t vneg = -(int32_t)(j & 1);      // odd clauses vote against
+  const uint32_t T2 = 2 * T;
+  const bool narrow = T < 32768u;               // u16 * 2T fits in 32 bits
+  // lanes past the row's end read its last tile instead (masked by every use)
+  const uint32_t wload = min(wbase, L.row_stride - WPL);
   uint32_t buf = 0;
   uint32_t n_steps = 0, n_t1 = 0, n_grow = 0, n_t2 = 0, n_t1z = 0, n_draws = 0;
 
@@ -332,7 +338,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
       nq_d = (dsel >> (16 * (j & 1))) & 0xFFFFu;
     };
     uint32_t xn[WPL];
-    auto fetch = [&](uint32_t sample) { load_words<WPL>(L.rows + (size_t) sample * L.row_stride, wbase, wd, xn); };
+    auto fetch = [&](uint32_t sample) { load_words<WPL>(L.rows + (size_t) sample * L.row_stride, wload, xn); };
 
     if (!scan()) continue;
     take();
@@ -361,13 +367,9 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
 #pragma unroll
       for (int k = 0; k < WPL; k++) f += __popc(st[M][k] & ~x[k] & word_mask(wbase + k, wd, L.last_eval));
       f = group_sum<G>(f);
-      int32_t votes;
-      if (lit == 0) votes = (int32_t) tol;
-      else {
-        votes = (int32_t)(lit < tol ? lit : tol) - (int32_t) f;
-        if (votes < 0) votes = 0;
-      }
-      if (g == 0) votes_lds[buf][j] = (j & 1) ? -votes : votes;
+      int32_t votes = cap - (int32_t) f;
+      if (votes < 0) votes = 0;
+      if (g == 0) votes_lds[buf][j] = (votes ^ vneg) - vneg;
       lds_barrier();
       int32_t v = 0;
 #pragma unroll
@@ -376,11 +378,13 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
       buf ^= 1;
       if (v > (int32_t) T) v = (int32_t) T;
       if (v < -(int32_t) T) v = -(int32_t) T;
-      const uint64_t num = (uint64_t)(tq ? (int64_t) T - v : (int64_t) T + v);
-      const bool fire = (uint64_t) u16 * (2ull * T) < (num << 16);
+      // fire iff u16 * 2T < (T -+ v) * 2^16; for integers x < num * 2^16
+      // exactly when x >> 16 < num, which stays in 32 bits when T < 2^15
+      const uint32_t num = (uint32_t)(tq ? (int32_t) T - v : (int32_t) T + v);
+      const bool fire = narrow ? ((u16 * T2) >> 16) < num : (uint64_t) u16 * T2 < ((uint64_t) num << 16);
       n_steps++;
       if (fire) {
-        const bool type1 = tq ? !(j & 1) : (j & 1);
+        const bool type1 = tq ^ (j & 1);
         const bool out = votes > 0;
         // kind 1: Type I on a true clause (capi.c:179-183): +1 on true
         //   literals when literals <= clause_maximum, -1 on false literals
@@ -477,7 +481,10 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
           } while (again);
           // with include_bits 1, a Type I step without growth only lowers
           // counters whose action bit is zero: the literal count stands
-          if (!(IB1 && kind == 1 && !grow)) lit = recount();
+          if (!(IB1 && kind == 1 && !grow)) {
+            lit = recount();
+            cap = cap_of(lit);
+          }
         }
       }
       if (!more) break;
