@@ -269,10 +269,28 @@ int stage_rows(tm_machine *m, const uint8_t *rows, uint32_t n, bool dev, uint32_
   return TM_OK;
 }
 
+// Train tile (lanes per clause x words per lane): the default for the row
+// width, or TSETLIN_AMD_TILE=<lanes>x<words> when that tile is built and
+// covers the row (for tuning; every tile gives the same machine).
+void train_tile(const tm_machine *m, uint32_t *lanes, uint32_t *wpl)
+{
+  *lanes = 64;
+  *wpl = 1;
+  tm_train_shape(m->wd, lanes, wpl);
+  const char *v = getenv("TSETLIN_AMD_TILE");
+  unsigned g = 0, w = 0;
+  if (v && sscanf(v, "%ux%u", &g, &w) == 2) {
+    uint32_t shapes[16][2];
+    const int n = tm_train_shapes(m->wd, shapes, 16);
+    for (int i = 0; i < n; i++)
+      if (shapes[i][0] == g && shapes[i][1] == w) { *lanes = g; *wpl = w; }
+  }
+}
+
 uint32_t train_row_stride(const tm_machine *m)
 {
-  uint32_t lanes = 64, wpl = 1;
-  tm_train_shape(m->wd, &lanes, &wpl);
+  uint32_t lanes, wpl;
+  train_tile(m, &lanes, &wpl);
   return (m->wd + wpl - 1) / wpl * wpl;
 }
 
@@ -523,6 +541,7 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
     L.last_live = m->last_live_mask();
     L.last_eval = m->last_eval_mask();
     L.all_ib1 = m->include_bits <= 1;
+    train_tile(m, &L.lanes, &L.wpl);
     std::vector<uint32_t> ek(4 * group);
     bool stop = false;
     for (uint32_t e0 = 0; e0 < iterations && !stop; e0 += group) {

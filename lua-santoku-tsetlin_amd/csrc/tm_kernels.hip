@@ -24,7 +24,9 @@ namespace {
 
 // Sum over the G consecutive lanes of a clause group, returned to every lane
 // of the group.  Full waves reduce with DPP row shifts and row broadcasts
-// (lane 63 ends with the total); smaller groups use butterfly shuffles.
+// (lane 63 ends with the total); half waves the same, with the broadcast
+// kept inside each half (lanes 31 and 63 end with the totals); smaller
+// groups use butterfly shuffles.
 template <int G>
 __device__ __forceinline__ uint32_t group_sum(uint32_t v)
 {
@@ -36,6 +38,15 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t v)
     v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xF, 0xF, true);   // row_bcast:15
     v += __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xF, 0xF, true);   // row_bcast:31
     return __builtin_amdgcn_readlane(v, 63);
+  }
+  if (G == 32) {
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, true);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, true);   // row_bcast:15 into rows 1, 3
+    const uint32_t lo = __builtin_amdgcn_readlane(v, 31), hi = __builtin_amdgcn_readlane(v, 63);
+    return (threadIdx.x & 32) ? hi : lo;
   }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);

@@ -186,6 +186,35 @@ def test_step_queue_matches_scan(features, classes, clauses, neg, n, monkeypatch
     assert got["1"][1]["steps"] == int(orc.stats[0])
 
 
+@pytest.mark.parametrize("kind", ["classifier", "encoder"])
+def test_every_tile_gives_the_same_machine(kind, monkeypatch):
+    """32 words per row: tiles 16x2, 32x1, 32x2, 64x1 and 64x2 (lanes x words)
+    all cover it; each must train the replay's machine bit for bit."""
+    rng = np.random.default_rng(77)
+    n = 300
+    rows = rows_for(512, n, rng, 0.3)
+    o = dict(features=512, classes=3, clauses=16, clause_tolerance=6, clause_maximum=8, specificity=16.0,
+             target=8, seed=55)
+    orc = None
+    for tile in ("16x2", "32x1", "32x2", "64x1", "64x2"):
+        monkeypatch.setenv("TSETLIN_AMD_TILE", tile)
+        eng, orc = pair(kind, **o)
+        if kind == "classifier":
+            labels = np.random.default_rng(5).integers(0, 3, n).astype(np.int64)
+            eng.train(samples=n, problems=rows, solutions=labels, iterations=2)
+        else:
+            codes = np.random.default_rng(5).integers(0, 8, (n, 1), dtype=np.uint8)
+            eng.train(samples=n, sentences=rows, codes=codes, iterations=2)
+        got = eng.checkpoint()
+        if tile == "16x2":
+            if kind == "classifier":
+                orc.train_replay(rows, labels=labels, epochs=2)
+            else:
+                orc.train_replay(rows, codes=codes, epochs=2)
+            want = orc.actions.copy()
+        assert np.array_equal(got, want), tile
+
+
 def test_each_callback_epochs_and_stop():
     rng = np.random.default_rng(6)
     rows = rows_for(64, 200, rng)
