@@ -428,19 +428,34 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
             uint32_t mk[WPL], sgn;
             if (kind == 2) {
               if (load) {
+                // first pass over the batch: every lane's draws in order
+                // (a trip count uniform over the group), the hash input
+                // stepping by G*phi; a draw whose literal is already hit
+                // this round stays pending
                 const uint32_t cnt = min(32u, nd - base);
-                for (uint32_t i = 0; i < cnt; i++)
-                  if ((base + i) * G + g < P.s_draws) pend |= 1u << i;
+                uint32_t r = base * G + g, hin = kev + r * TM_GOLDEN;
+                const uint32_t hstep = G * TM_GOLDEN;
+#pragma unroll 4
+                for (uint32_t i = 0; i < cnt; i++) {
+                  const bool valid = r < P.s_draws;
+                  const uint32_t lbit = tm_mulhi(tm_lowbias32(hin), L.input_bits);
+                  const uint32_t hb = valid ? 1u << (lbit & 31) : 0u;
+                  const uint32_t old = atomicOr(&hits[j][lbit >> 5], hb);
+                  if (old & hb) pend |= 1u << i;
+                  r += G;
+                  hin += hstep;
+                }
                 load = false;
-              }
-              uint32_t pp = pend;
-              while (pp) {
-                const uint32_t i = __builtin_ctz(pp);
-                pp &= pp - 1;
-                const uint32_t lbit = tm_literal_draw(kev, (base + i) * G + g, L.input_bits);
-                const uint32_t hb = 1u << (lbit & 31);
-                const uint32_t old = atomicOr(&hits[j][lbit >> 5], hb);
-                if (!(old & hb)) pend &= ~(1u << i);
+              } else {
+                uint32_t pp = pend;
+                while (pp) {
+                  const uint32_t i = __builtin_ctz(pp);
+                  pp &= pp - 1;
+                  const uint32_t lbit = tm_literal_draw(kev, (base + i) * G + g, L.input_bits);
+                  const uint32_t hb = 1u << (lbit & 31);
+                  const uint32_t old = atomicOr(&hits[j][lbit >> 5], hb);
+                  if (!(old & hb)) pend &= ~(1u << i);
+                }
               }
               __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
 #pragma unroll
