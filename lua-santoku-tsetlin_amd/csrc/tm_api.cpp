@@ -527,6 +527,14 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
     P.seed_lo = (uint32_t) m->seed;
     P.seed_hi = (uint32_t)(m->seed >> 32);
     P.active = 1;
+    P.rows = words;
+    P.row_stride = stride;
+    P.wd = m->wd;
+    P.input_bits = m->input_bits;
+    P.last_live = m->last_live_mask();
+    P.last_eval = m->last_eval_mask();
+    P.class_base = 0;
+    P.chunk_base = 0;
     L.rows = words;
     L.labels = dlabels;
     L.codes = dcodes;
@@ -1060,6 +1068,14 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
     P.seed_lo = (uint32_t) m->seed;
     P.seed_hi = (uint32_t)(m->seed >> 32);
     P.active = 1;
+    P.rows = words;
+    P.row_stride = stride;
+    P.wd = m->wd;
+    P.input_bits = m->input_bits;
+    P.last_live = m->last_live_mask();
+    P.last_eval = m->last_eval_mask();
+    P.class_base = 0;
+    P.chunk_base = 0;
   }
   HIP_TRY(hipMemcpyAsync(b->ekeys.p, ek.data(), ek.size() * 4, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(tm_launch_zero(b->stats.p, (size_t) nt * TM_NSTATS * 8, b->stream));

@@ -26,6 +26,12 @@ struct tm_train_trial {
   uint32_t tolerance, maximum, target, s_draws, include_bits;
   uint32_t seed_lo, seed_hi;
   uint32_t active;                // 0: the trial is frozen (its workgroups return at once)
+  const uint32_t *rows;           // literal rows of this trial [n][row_stride]
+  uint32_t row_stride, wd;        // words between rows (a multiple of the tile's words per lane), per row
+  uint32_t input_bits;            // literal bits (the range of the random-literal draws)
+  uint32_t last_live, last_eval;  // masks of the last word (Wd-1)
+  uint32_t class_base;            // class (label / code bit) of the trial's chunk 0
+  uint32_t chunk_base;            // index of the trial's chunk 0 in the random stream
 };
 
 struct tm_train_launch {

@@ -163,6 +163,21 @@ __device__ __forceinline__ void load_words(const uint32_t *row, uint32_t wbase, 
   }
 }
 
+// A workgroup-uniform struct copied word by word through readfirstlane: the
+// trial's parameters then live in scalar registers (loaded from either the
+// trial table or the kernel arguments, the compiler would keep them in VGPRs).
+template <typename T>
+__device__ __forceinline__ T to_sgpr(const T &v)
+{
+  static_assert(sizeof(T) % 4 == 0, "32-bit words");
+  T out;
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(&v);
+  uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+  for (int i = 0; i < (int) (sizeof(T) / 4); i++) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return out;
+}
+
 // ---------------------------------------------------------------------------
 // training
 
@@ -185,14 +200,16 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
     trial = L.block_map[2 * blockIdx.x];
     ch = L.block_map[2 * blockIdx.x + 1];
   }
-  const tm_train_trial P = L.trials ? L.trials[trial] : L.one;
+  const tm_train_trial P = to_sgpr(L.trials ? L.trials[trial] : L.one);
   if (!P.active) return;
   const uint32_t epoch_first = L.epoch_dev ? *L.epoch_dev : L.epoch_first;
   const bool init = L.epoch_dev ? epoch_first == 0 : L.init != 0;
-  const uint32_t cls = ch / P.clause_chunks;
-  const uint32_t clause = ch * 8 + j;
+  const uint32_t cls = P.class_base + ch / P.clause_chunks;
+  const uint32_t clause = ch * 8 + j;                 // index into this trial's state
+  const uint32_t chr = P.chunk_base + ch;             // chunk and clause in the random stream
+  const uint32_t clause_r = chr * 8 + j;
   const uint32_t wbase = g * WPL;
-  const uint32_t wd = L.wd;
+  const uint32_t wd = P.wd;
   uint32_t *counts = P.counts;
   uint32_t *actions = P.actions;
 
@@ -222,7 +239,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
   auto recount = [&]() -> uint32_t {
     uint32_t c = 0;
 #pragma unroll
-    for (int k = 0; k < WPL; k++) c += __popc(st[M][k] & word_mask(wbase + k, wd, L.last_eval));
+    for (int k = 0; k < WPL; k++) c += __popc(st[M][k] & word_mask(wbase + k, wd, P.last_eval));
     return group_sum<G>(c);
   };
   uint32_t lit = recount();
@@ -239,7 +256,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
   const uint32_t T2 = 2 * T;
   const bool narrow = T < 32768u;               // u16 * 2T fits in 32 bits
   // lanes past the row's end read its last tile instead (masked by every use)
-  const uint32_t wload = min(wbase, L.row_stride - WPL);
+  const uint32_t wload = min(wbase, P.row_stride - WPL);
   uint32_t buf = 0;
   uint32_t n_steps = 0, n_t1 = 0, n_grow = 0, n_t2 = 0, n_t1z = 0, n_draws = 0;
 
@@ -309,14 +326,14 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
           } else {
             const uint32_t sc = valid ? (uint32_t) L.labels[s] : 0xFFFFFFFFu;
             tgt = valid && sc == cls;
-            w = valid && (tgt || (uint64_t) tm_skip_draw(ek0, ek1, s, ch) >= P.skip_threshold);
+            w = valid && (tgt || (uint64_t) tm_skip_draw(ek0, ek1, s, chr) >= P.skip_threshold);
           }
           b0 += 64;
           want = __ballot(w);
           if (want) {
             tmask = __ballot(tgt);
             bs = s;
-            bd = tm_philox(s, ch, epoch, TM_TAG_STEP, P.seed_lo, P.seed_hi);
+            bd = tm_philox(s, chr, epoch, TM_TAG_STEP, P.seed_lo, P.seed_hi);
             if constexpr (G == 64) bsel = (j >> 1) == 0 ? bd.x : (j >> 1) == 1 ? bd.y : (j >> 1) == 2 ? bd.z : bd.w;
             return true;
           }
@@ -349,7 +366,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
       nq_d = (dsel >> (16 * (j & 1))) & 0xFFFFu;
     };
     uint32_t xn[WPL];
-    auto fetch = [&](uint32_t sample) { load_words<WPL>(L.rows + (size_t) sample * L.row_stride, wload, xn); };
+    auto fetch = [&](uint32_t sample) { load_words<WPL>(P.rows + (size_t) sample * P.row_stride, wload, xn); };
 
     if (!scan()) continue;
     take();
@@ -376,7 +393,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
 
       uint32_t f = 0;
 #pragma unroll
-      for (int k = 0; k < WPL; k++) f += __popc(st[M][k] & ~x[k] & word_mask(wbase + k, wd, L.last_eval));
+      for (int k = 0; k < WPL; k++) f += __popc(st[M][k] & ~x[k] & word_mask(wbase + k, wd, P.last_eval));
       f = group_sum<G>(f);
       int32_t votes = cap - (int32_t) f;
       if (votes < 0) votes = 0;
@@ -417,7 +434,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
             // decrement to every distinct literal hit by the draws still
             // pending; a draw whose literal was already hit this round waits
             // for the next round.
-            kev = tm_literal_key(ek2, ek3, sq, clause);
+            kev = tm_literal_key(ek2, ek3, sq, clause_r);
             nd = (P.s_draws + G - 1) / G;
             n_t1z++;
             n_draws += P.s_draws;
@@ -438,7 +455,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
 #pragma unroll 4
                 for (uint32_t i = 0; i < cnt; i++) {
                   const bool valid = r < P.s_draws;
-                  const uint32_t lbit = tm_mulhi(tm_lowbias32(hin), L.input_bits);
+                  const uint32_t lbit = tm_mulhi(tm_lowbias32(hin), P.input_bits);
                   const uint32_t hb = valid ? 1u << (lbit & 31) : 0u;
                   const uint32_t old = atomicOr(&hits[j][lbit >> 5], hb);
                   if (old & hb) pend |= 1u << i;
@@ -451,7 +468,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
                 while (pp) {
                   const uint32_t i = __builtin_ctz(pp);
                   pp &= pp - 1;
-                  const uint32_t lbit = tm_literal_draw(kev, (base + i) * G + g, L.input_bits);
+                  const uint32_t lbit = tm_literal_draw(kev, (base + i) * G + g, P.input_bits);
                   const uint32_t hb = 1u << (lbit & 31);
                   const uint32_t old = atomicOr(&hits[j][lbit >> 5], hb);
                   if (!(old & hb)) pend &= ~(1u << i);
@@ -468,21 +485,21 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
               // tk_automata_dec_not_excluded (capi.c:181-182)
 #pragma unroll
               for (int k = 0; k < WPL; k++)
-                mk[k] = x[k] & ~all_ones<M, WPL>(st, k) & word_mask(wbase + k, wd, L.last_live);
+                mk[k] = x[k] & ~all_ones<M, WPL>(st, k) & word_mask(wbase + k, wd, P.last_live);
               sgn = ~0u;
             } else if (kind == 3) {
               // +1 on false literals that are not included; none is at the
               // bound, since a counter with every plane one is included
 #pragma unroll
               for (int k = 0; k < WPL; k++)
-                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & word_mask(wbase + k, wd, L.last_live);
+                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & word_mask(wbase + k, wd, P.last_live);
               sgn = ~0u;
             } else {
               // -1 on false literals that are not included and not at zero
               // (with include_bits 1 their action plane is zero already)
 #pragma unroll
               for (int k = 0; k < WPL; k++)
-                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & word_mask(wbase + k, wd, L.last_live) &
+                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & word_mask(wbase + k, wd, P.last_live) &
                         any_one<M, WPL, IB1 ? M - 1 : M>(st, k);
               sgn = 0u;
             }
@@ -813,6 +830,7 @@ __global__ __launch_bounds__(256) void queue_write_kernel(tm_queue_launch Q)
 template <int WPL, int G>
 int launch_train_shape(const tm_train_launch &L, hipStream_t st)
 {
+  // L.wd is the widest trial's row; the host checks each trial's stride
   if (L.state_bits != 8 || L.row_stride % WPL != 0 || L.row_stride < L.wd || (uint64_t) G * WPL < L.wd)
     return (int) hipErrorInvalidValue;
   // one workgroup per chunk: the grid must stay below 2^32 work-items
