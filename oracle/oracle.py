@@ -96,6 +96,11 @@ def _bind(L):
         "orc_train_refshape": (None, [P, _U8P, _U8P, ctypes.c_uint32, _U8P, _I64P, _U8P,
                                       ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32]),
         "orc_max_threads": (ctypes.c_int, []),
+        "orc_ind_class": (None, [P, ctypes.c_uint64, P]),
+        "orc_train_replay_ind": (None, [P, _U64P, _U8P, _U8P, ctypes.c_uint32, _U8P, _I64P, _I64P, _U8P,
+                                        ctypes.c_uint32, ctypes.c_uint32, _U64P]),
+        "orc_class_sums_ind": (None, [P, _U64P, _U8P, _U8P, _I64P, ctypes.c_uint32, _I64P]),
+        "orc_predict_ind": (None, [P, _U64P, _U8P, _U8P, _I64P, ctypes.c_uint32, _I64P, _U8P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -179,6 +184,57 @@ class Machine:
         n = rows.shape[0]
         out = np.zeros((n, self.p.classes), np.int64)
         lib().orc_class_sums(ctypes.byref(self.p), ptr(self.actions), ptr(rows), n, ptr(out, _I64P))
+        return out
+
+
+class IndMachine:
+    """Oracle individualized machine (capi.c:528-606): class h is a one-class
+    machine over feat_sizes[h] features; counts and actions of the classes
+    sit one after another in class order, the reference's layout."""
+
+    def __init__(self, p, feat_sizes):
+        self.p = p
+        self.feat_sizes = np.ascontiguousarray(feat_sizes, np.uint64)
+        self.chunks = [(2 * int(k) + 7) // 8 for k in self.feat_sizes]
+        self.counts = np.zeros(sum(p.clauses * (p.state_bits - 1) * b for b in self.chunks), np.uint8)
+        self.actions = np.zeros(sum(p.clauses * b for b in self.chunks), np.uint8)
+        self.stats = np.zeros(6, np.uint64)
+
+    def action_slices(self):
+        out, o = [], 0
+        for b in self.chunks:
+            out.append(slice(o, o + self.p.clauses * b))
+            o += self.p.clauses * b
+        return out
+
+    def train_replay(self, problems, dim_offsets, n, labels=None, codes=None, epochs=1, epoch_first=0):
+        prob = np.ascontiguousarray(problems, np.uint8)
+        offs = np.ascontiguousarray(dim_offsets, np.int64)
+        lab = None if labels is None else np.ascontiguousarray(labels, np.int64)
+        cod = None if codes is None else np.ascontiguousarray(codes, np.uint8)
+        lib().orc_train_replay_ind(ctypes.byref(self.p), ptr(self.feat_sizes, _U64P), ptr(self.counts),
+                                   ptr(self.actions), n, ptr(prob), ptr(offs, _I64P), ptr(lab, _I64P), ptr(cod),
+                                   epoch_first, epochs, ptr(self.stats, _U64P))
+
+    def class_sums(self, problems, dim_offsets, n):
+        prob = np.ascontiguousarray(problems, np.uint8)
+        offs = np.ascontiguousarray(dim_offsets, np.int64)
+        out = np.zeros((n, self.p.classes), np.int64)
+        lib().orc_class_sums_ind(ctypes.byref(self.p), ptr(self.feat_sizes, _U64P), ptr(self.actions), ptr(prob),
+                                 ptr(offs, _I64P), n, ptr(out, _I64P))
+        return out
+
+    def predict(self, problems, dim_offsets, n):
+        prob = np.ascontiguousarray(problems, np.uint8)
+        offs = np.ascontiguousarray(dim_offsets, np.int64)
+        if self.p.type == 0:
+            out = np.zeros(n, np.int64)
+            lib().orc_predict_ind(ctypes.byref(self.p), ptr(self.feat_sizes, _U64P), ptr(self.actions), ptr(prob),
+                                  ptr(offs, _I64P), n, ptr(out, _I64P), None)
+        else:
+            out = np.zeros((n, self.p.class_chunks), np.uint8)
+            lib().orc_predict_ind(ctypes.byref(self.p), ptr(self.feat_sizes, _U64P), ptr(self.actions), ptr(prob),
+                                  ptr(offs, _I64P), n, None, ptr(out))
         return out
 
 

@@ -505,8 +505,12 @@ void orc_predict_encoder (const orc_params *p, const uint8_t *actions, const uin
  * events, [4] random-literal draws, [5] negative-sampling skips. */
 #define ORC_NSTATS 6
 
+/* One chunk-step.  `chunk` indexes the state arrays; `chunk_r` is the chunk's
+ * index in the random stream (equal for a plain machine; for an
+ * individualized one, class h's chunk c is chunk h * clause_chunks + c of the
+ * stream while its state is class h's own array). */
 static void replay_step (const orc_params *p, uint8_t *counts, uint8_t *actions, const uint8_t *x,
-                         uint32_t sample, uint32_t chunk, uint32_t epoch, bool tgt,
+                         uint32_t sample, uint32_t chunk, uint32_t chunk_r, uint32_t epoch, bool tgt,
                          const uint32_t ek[4], uint32_t s_draws, uint64_t *st)
 {
   uint32_t lit[8], vot[8];
@@ -518,7 +522,7 @@ static void replay_step (const orc_params *p, uint8_t *counts, uint8_t *actions,
   uint64_t num = (uint64_t)(tgt ? T - v : T + v);                   /* capi.c:221-225 */
   uint64_t den = 2 * (uint64_t) T;
   uint32_t d[4];
-  orc_step_draws(p->seed, epoch, sample, chunk, d);
+  orc_step_draws(p->seed, epoch, sample, chunk_r, d);
   st[0]++;
   for (uint32_t j = 0; j < 8; j++) {                                /* capi.c:226-242 */
     uint64_t u = (d[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
@@ -533,7 +537,7 @@ static void replay_step (const orc_params *p, uint8_t *counts, uint8_t *actions,
         orc_dec_not_excluded(p, counts, actions, clause, x);
         st[1]++;
       } else {                                                      /* capi.c:184-192 */
-        uint32_t kev = orc_literal_key(ek, sample, (uint32_t) clause);
+        uint32_t kev = orc_literal_key(ek, sample, chunk_r * 8 + j);
         for (uint32_t r = 0; r < s_draws; r++) {
           uint32_t bit = orc_literal_draw(kev, r, p->input_bits);
           orc_dec_byte(p, counts, actions, clause, bit / 8, (uint8_t)(1u << (bit % 8)));
@@ -593,7 +597,7 @@ void orc_train_replay (const orc_params *p, uint8_t *counts, uint8_t *actions,
           } else {
             tgt = (codes[(uint64_t) s * p->class_chunks + cls / 8] >> (cls % 8)) & 1;
           }
-          replay_step(p, counts, actions, x, s, ch, e, tgt, ek, s_draws, lst);
+          replay_step(p, counts, actions, x, s, ch, ch, e, tgt, ek, s_draws, lst);
         }
       }
       #pragma omp critical
@@ -737,6 +741,147 @@ void orc_predict_encoder_threads (const orc_params *p, const uint8_t *actions, c
   if (nthreads > 0) omp_set_num_threads(nthreads);
   orc_predict_encoder(p, actions, rows, n, out);
   omp_set_num_threads(prev);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Individualized machines (capi.c:258-402, :528-626, :997-1086, :1284-1482) */
+/* ------------------------------------------------------------------------ */
+
+/* Class h of an individualized machine is a one-class machine over its own
+ * feat_sizes[h] features (row width, tail mask and random-draw range of its
+ * own, capi.c:576-604) sharing clauses, tolerance, maximum, target,
+ * specificity, negative sampling and seed with the others.  Its counts and
+ * actions sit one class after another, in class order (capi.c:585-588). */
+void orc_ind_class (const orc_params *p, uint64_t k_h, orc_params *out)
+{
+  *out = *p;
+  out->classes = 1;
+  out->features = (uint32_t) k_h;
+  orc_derive(out);
+  out->target = p->target;        /* target and negative are the machine's (capi.c:555-566) */
+  out->negative = p->negative;
+  out->class_chunks = p->class_chunks;
+}
+
+static void ind_offsets (const orc_params *p, const uint64_t *feat_sizes, uint32_t h, uint64_t *coff, uint64_t *aoff)
+{
+  uint64_t c = 0, a = 0;
+  for (uint32_t i = 0; i < h; i++) {
+    uint64_t b = bits_bytes(2 * feat_sizes[i]);
+    c += (uint64_t) p->clauses * b * (p->state_bits - 1);
+    a += (uint64_t) p->clauses * b;
+  }
+  *coff = c;
+  *aoff = a;
+}
+
+/* Training (capi.c:1284-1482) with the engine's stream: one epoch order for
+ * every chunk; class h's chunk c draws as chunk h * clause_chunks + c.
+ * Sample s of class h reads problems + dim_offsets[h] + s * input_chunks_h.
+ * Classifier: negative sampling as the plain machine; encoder: every sample
+ * (capi.c:1354 passes 0.0). */
+void orc_train_replay_ind (const orc_params *p, const uint64_t *feat_sizes, uint8_t *counts, uint8_t *actions,
+                           uint32_t n, const uint8_t *problems, const int64_t *dim_offsets,
+                           const int64_t *labels, const uint8_t *codes, uint32_t epoch_first, uint32_t epochs,
+                           uint64_t *stats)
+{
+  const uint32_t K = p->classes, cc = p->clause_chunks;
+  if (epoch_first == 0)
+    for (uint32_t h = 0; h < K; h++) {
+      orc_params ph;
+      orc_ind_class(p, feat_sizes[h], &ph);
+      uint64_t co, ao;
+      ind_offsets(p, feat_sizes, h, &co, &ao);
+      orc_setup(&ph, counts + co, actions + ao, 0, (uint64_t) cc * 8 - 1);
+    }
+  uint32_t *perm = malloc((size_t) n * sizeof(uint32_t) + 4);
+  uint64_t thr = orc_threshold32(1.0 - p->negative);
+  uint64_t st[ORC_NSTATS] = { 0 };
+  for (uint32_t e = epoch_first; e < epoch_first + epochs; e++) {
+    orc_perm(p->seed, e, n, perm);
+    uint32_t ek[4];
+    orc_epoch_keys(p->seed, e, ek);
+    #pragma omp parallel
+    {
+      uint64_t lst[ORC_NSTATS] = { 0 };
+      #pragma omp for schedule(dynamic, 1)
+      for (uint32_t chg = 0; chg < K * cc; chg++) {
+        const uint32_t h = chg / cc, c = chg % cc;
+        orc_params ph;
+        orc_ind_class(p, feat_sizes[h], &ph);
+        uint64_t co, ao;
+        ind_offsets(p, feat_sizes, h, &co, &ao);
+        const uint32_t s_draws = orc_s_draws(&ph);
+        for (uint32_t i = 0; i < n; i++) {
+          uint32_t s = perm[i];
+          const uint8_t *x = problems + dim_offsets[h] + (uint64_t) s * ph.input_chunks;
+          bool tgt;
+          if (p->type == 0) {
+            tgt = (uint32_t) labels[s] == h;
+            if (!tgt && (uint64_t) orc_skip_draw(ek, s, chg) < thr) { lst[5]++; continue; }
+          } else {
+            tgt = (codes[(uint64_t) s * p->class_chunks + h / 8] >> (h % 8)) & 1;
+          }
+          replay_step(&ph, counts + co, actions + ao, x, s, c, chg, e, tgt, ek, s_draws, lst);
+        }
+      }
+      #pragma omp critical
+      for (int k = 0; k < ORC_NSTATS; k++) st[k] += lst[k];
+    }
+  }
+  if (stats) for (int k = 0; k < ORC_NSTATS; k++) stats[k] = st[k];
+  free(perm);
+}
+
+/* Class sums of an individualized machine (capi.c:1010-1029, :1057-1073):
+ * class h evaluates its own input row. */
+void orc_class_sums_ind (const orc_params *p, const uint64_t *feat_sizes, const uint8_t *actions,
+                         const uint8_t *problems, const int64_t *dim_offsets, uint32_t n, int64_t *sums)
+{
+  const uint32_t K = p->classes, cc = p->clause_chunks;
+  #pragma omp parallel for schedule(static)
+  for (uint32_t s = 0; s < n; s++) {
+    uint32_t lit[8], vot[8];
+    for (uint32_t h = 0; h < K; h++) {
+      orc_params ph;
+      orc_ind_class(p, feat_sizes[h], &ph);
+      uint64_t co, ao;
+      ind_offsets(p, feat_sizes, h, &co, &ao);
+      const uint8_t *x = problems + dim_offsets[h] + (uint64_t) s * ph.input_chunks;
+      int64_t sum = 0;
+      for (uint32_t c = 0; c < cc; c++) {
+        uint8_t out = orc_calculate(&ph, actions + ao, x, c, lit, vot);
+        sum += orc_sums(out, vot);
+      }
+      sums[(uint64_t) s * K + h] = sum;
+    }
+  }
+}
+
+/* tk_tsetlin_predict_classifier_ind / _encoder_ind (capi.c:997-1086): argmax
+ * (strict '>', ties to the lowest class) or bit h = sum_h > 0. */
+void orc_predict_ind (const orc_params *p, const uint64_t *feat_sizes, const uint8_t *actions,
+                      const uint8_t *problems, const int64_t *dim_offsets, uint32_t n, int64_t *out_class,
+                      uint8_t *out_codes)
+{
+  const uint32_t K = p->classes;
+  int64_t *sums = malloc((size_t) n * K * sizeof(int64_t) + 8);
+  orc_class_sums_ind(p, feat_sizes, actions, problems, dim_offsets, n, sums);
+  for (uint32_t s = 0; s < n; s++) {
+    const int64_t *row = sums + (uint64_t) s * K;
+    if (out_class) {
+      int64_t best = -INT64_MAX;
+      uint32_t bc = 0;
+      for (uint32_t h = 0; h < K; h++) if (row[h] > best) { best = row[h]; bc = h; }
+      out_class[s] = bc;
+    }
+    if (out_codes) {
+      uint8_t *e = out_codes + (uint64_t) s * p->class_chunks;
+      memset(e, 0, p->class_chunks);
+      for (uint32_t h = 0; h < K; h++) if (row[h] > 0) e[h / 8] |= (uint8_t)(1u << (h % 8));
+    }
+  }
+  free(sums);
 }
 
 int orc_max_threads (void) { return omp_get_max_threads(); }
