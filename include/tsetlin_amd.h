@@ -58,7 +58,8 @@ typedef struct {
   double negative;           /* default -1: 1/classes */
   double specificity;        /* required, > 0 */
   int32_t reusable;          /* default 0 */
-  int32_t individualized;    /* default 0; 1 is TM_E_UNSUPPORTED in this build */
+  int32_t individualized;    /* default 0; individualized machines are created
+                                by tm_create_*_ind (1 here is TM_E_ARG) */
   uint64_t seed;             /* RNG seed of train (no counterpart: the reference
                                 seeds each OpenMP thread with its thread id) */
 } tm_options;
@@ -117,6 +118,64 @@ int tm_predict_encoder(tm_machine *m, const uint8_t *problems, uint32_t n, uint8
 int tm_predict_classifier_dev(tm_machine *m, const uint8_t *problems_dev, uint32_t n, int64_t *out_dev);
 int tm_predict_encoder_dev(tm_machine *m, const uint8_t *problems_dev, uint32_t n, uint8_t *out_dev);
 
+/* ---------------------------------------------------------------------------
+ * Individualized machines: class h (hidden bit h of an encoder) has a feature
+ * set of its own, k_h = feat_offsets[h + 1] - feat_offsets[h] features, and
+ * is a one-class machine over rows of ceil(2 k_h / 8) bytes.  The plain
+ * train/predict entry points refuse these machines and the _ind ones refuse
+ * plain machines; persist, load, checkpoint, restore, reconfigure, type and
+ * destroy work on both.
+ *
+ * Problems of an individualized call: one buffer of problems_bytes bytes;
+ * class h's row of sample s is the ceil(2 k_h / 8) bytes at byte
+ * dim_offsets[h] + s * ceil(2 k_h / 8) (classes entries in dim_offsets).
+ * TM_E_SIZE if some class's rows run past problems_bytes.
+ */
+
+/* create("classifier", {individualized = true, feat_offsets, ...})
+ * -> tk_tsetlin_init_classifier_ind, capi.c:528-606 (dispatch capi.c:631-645).
+ * n_offsets must be classes + 1; opts->features and opts->individualized are
+ * not read.  An empty feature set (k_h = 0) is TM_E_ARG. */
+int tm_create_classifier_ind(const tm_options *opts, const int64_t *feat_offsets, uint32_t n_offsets,
+                             tm_machine **out);
+
+/* create("encoder", {individualized = true, feat_offsets, hidden, ...})
+ * -> tk_tsetlin_init_encoder_ind, capi.c:608-626 (dispatch capi.c:668-682). */
+int tm_create_encoder_ind(const tm_options *opts, const int64_t *feat_offsets, uint32_t n_offsets,
+                          tm_machine **out);
+
+/* machine:train{samples, problems, solutions, dim_offsets, iterations, each}
+ * -> tk_tsetlin_train_classifier_ind, capi.c:1387-1482. */
+int tm_train_classifier_ind(tm_machine *m, uint32_t n, const uint8_t *problems, size_t problems_bytes,
+                            const int64_t *solutions, const int64_t *dim_offsets, uint32_t iterations,
+                            tm_epoch_fn each, void *user);
+
+/* machine:train{samples, sentences, codes, dim_offsets, iterations, each}
+ * -> tk_tsetlin_train_encoder_ind, capi.c:1284-1385. */
+int tm_train_encoder_ind(tm_machine *m, uint32_t n, const uint8_t *sentences, size_t sentences_bytes,
+                         const uint8_t *codes, const int64_t *dim_offsets, uint32_t iterations, tm_epoch_fn each,
+                         void *user);
+
+/* machine:predict(problems, dim_offsets, n) -> tk_tsetlin_predict_classifier_ind,
+ * capi.c:997-1043: n int64 class ids. */
+int tm_predict_classifier_ind(tm_machine *m, const uint8_t *problems, size_t problems_bytes,
+                              const int64_t *dim_offsets, uint32_t n, int64_t *out);
+
+/* machine:predict(problems, dim_offsets, n) -> tk_tsetlin_predict_encoder_ind,
+ * capi.c:1045-1086: n * ceil(hidden/8) bytes. */
+int tm_predict_encoder_ind(tm_machine *m, const uint8_t *problems, size_t problems_bytes,
+                           const int64_t *dim_offsets, uint32_t n, uint8_t *out);
+
+/* machine:restrict(selected_dims) -> tk_tsetlin_restrict, capi.c:752-900:
+ * keep the classes listed in `selected` (in that order), with their actions
+ * and, while the machine has them, counts.  Errors as the reference: not
+ * individualized, empty, out of range, duplicate (all TM_E_ARG). */
+int tm_restrict(tm_machine *m, const int64_t *selected, uint32_t n_selected);
+
+/* Feature sizes k_h of an individualized machine (the feat_sizes the
+ * reference persists, capi.c:1528-1530); TM_E_SIZE if cap < classes. */
+int tm_get_feature_sizes(const tm_machine *m, uint64_t *out, uint32_t cap);
+
 /* machine:persist(nil) -> tk_tsetlin_persist, capi.c:1538-1562 (layout
  * capi.c:1509-1531).  Writes the persisted bytes into `buf` when cap is large
  * enough; *len always receives the required size.  TM_E_SIZE if cap < *len. */
@@ -160,7 +219,7 @@ typedef struct {
   uint32_t type, classes, features, clauses, clause_tolerance, clause_maximum, target, state_bits,
            include_bits, input_bits, input_chunks, clause_chunks, state_chunks, action_chunks, class_chunks;
   double specificity, negative;
-  uint8_t tail_mask, reusable, has_state, trained;
+  uint8_t tail_mask, reusable, has_state, trained, individualized;
 } tm_info;
 int tm_get_info(const tm_machine *m, tm_info *out);
 

@@ -66,7 +66,41 @@ struct dbuf {
   template <typename T> T *as() const { return (T *) p; }
 };
 
+// Literal bits < input_bits inside the last word of a row of wd words.
+uint32_t eval_mask(uint32_t wd, uint32_t input_bits)
+{
+  uint32_t bits = input_bits - 32 * (wd - 1);
+  return bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+}
+
+// Bits of the last word whose automata evolve.  Bytes < input_chunks are live.
+// The last byte keeps only its tail bits when the reference's masked byte path
+// handles it (input_chunks % 16 != 0, automata.h:97,106); inside the unmasked
+// 128-bit bulk region all 8 bits of the byte evolve (automata.h:164-194).
+uint32_t live_mask(uint32_t wd, uint32_t input_chunks, uint8_t tail_mask)
+{
+  uint32_t m = 0;
+  for (uint32_t b = 0; b < 4; b++) {
+    uint32_t byte = 4 * (wd - 1) + b;
+    if (byte >= input_chunks) break;
+    uint8_t bm = 0xFF;
+    if (byte == input_chunks - 1 && (input_chunks % 16) != 0) bm = tail_mask;
+    m |= (uint32_t) bm << (8 * b);
+  }
+  return m;
+}
+
 } // namespace
+
+// One class of an individualized machine: a one-class machine over its own
+// feature set (capi.c:576-604), stored after the classes before it.
+struct tm_ind_class {
+  uint64_t features = 0;                  // k_h
+  uint32_t input_bits = 0, input_chunks = 0, wd = 0;
+  uint8_t tail_mask = 0xFF;
+  uint64_t aoff = 0, coff = 0;            // u32-word offsets of its actions and counts
+  uint64_t hoff = 0;                      // byte offset of its actions in the persisted layout
+};
 
 struct tm_machine {
   int type = TM_CLASSIFIER;
@@ -77,16 +111,18 @@ struct tm_machine {
   uint8_t tail_mask = 0xFF;
   double negative = 0, specificity = 0;
   uint64_t seed = 0;
+  std::vector<tm_ind_class> ind;    // individualized: one entry per class
 
   int device = 0;
   hipStream_t stream = nullptr;
   bool owns_stream = true;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  uint32_t wd = 0;                  // u32 words per literal row
+  uint32_t wd = 0;                  // u32 words per literal row (individualized: the widest class)
   dbuf counts, actions;             // automata state
   dbuf actT, lit;                   // predict-side views of the include plane
   bool predict_ready = false;
   dbuf t_rows, t_words, t_labels, t_codes, t_perm, t_perm_scratch, t_ekeys;   // training data
+  dbuf t_trials, t_map;                                                       // individualized train table
   dbuf p_rows, p_words, p_sums, p_out;                                        // predict data
   dbuf t_stats;
   dbuf q_counts, q_offsets, q_scratch, q_id, q_draw;                          // step queue
@@ -96,28 +132,16 @@ struct tm_machine {
   double train_ms = 0, predict_ms = 0, queue_ms = 0;
 
   uint64_t nclauses() const { return (uint64_t) classes * clauses; }
-  uint32_t last_eval_mask() const
+  // u32 words of the include plane (the counts hold state_bits - 1 times as many)
+  uint64_t action_words() const
   {
-    // literal bits < input_bits inside the last word
-    uint32_t bits = input_bits - 32 * (wd - 1);
-    return bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1);
+    if (!individualized) return nclauses() * wd;
+    uint64_t w = 0;
+    for (const tm_ind_class &c : ind) w += (uint64_t) clauses * c.wd;
+    return w;
   }
-  uint32_t last_live_mask() const
-  {
-    // Bytes < input_chunks are live.  The last byte keeps only its tail bits
-    // when the reference's masked byte path handles it (input_chunks % 16 != 0,
-    // automata.h:97,106); inside the unmasked 128-bit bulk region all 8 bits
-    // of the byte evolve (automata.h:164-194).
-    uint32_t m = 0;
-    for (uint32_t b = 0; b < 4; b++) {
-      uint32_t byte = 4 * (wd - 1) + b;
-      if (byte >= input_chunks) break;
-      uint8_t bm = 0xFF;
-      if (byte == input_chunks - 1 && (input_chunks % 16) != 0) bm = tail_mask;
-      m |= (uint32_t) bm << (8 * b);
-    }
-    return m;
-  }
+  uint32_t last_eval_mask() const { return eval_mask(wd, input_bits); }
+  uint32_t last_live_mask() const { return live_mask(wd, input_chunks, tail_mask); }
 };
 
 namespace {
@@ -146,18 +170,20 @@ uint32_t derive_target(double targetf, uint32_t clauses, uint32_t tolerance)
   return (uint32_t) t;
 }
 
-int validate_opts(const tm_options *o, const char *what)
+// capi.c:463-472 (plain) and :542-553 (individualized: no features option;
+// the feature sets come from feat_offsets).
+int validate_opts(const tm_options *o, const char *what, bool ind = false)
 {
   if (!o) return fail(TM_E_ARG, "%s: options are NULL", what);
-  // capi.c:463-472
   if (!o->classes) return fail(TM_E_ARG, "%s: classes must be greater than 1", what);
   if (!o->clauses) return fail(TM_E_ARG, "%s: clauses must be greater than 0", what);
   if (!o->clause_tolerance) return fail(TM_E_ARG, "%s: clause_tolerance must be greater than 0", what);
   if (!o->clause_maximum) return fail(TM_E_ARG, "%s: clause_maximum must be greater than 0", what);
   if (o->state < 2) return fail(TM_E_ARG, "%s: bits must be greater than 1", what);
   if (!(o->specificity > 0)) return fail(TM_E_ARG, "%s: specificity must be a positive number", what);
-  if (!o->features) return fail(TM_E_ARG, "%s: features must be greater than 0", what);
-  if (o->individualized) return fail(TM_E_UNSUPPORTED, "%s: individualized machines are not built in this engine", what);
+  if (!ind && !o->features) return fail(TM_E_ARG, "%s: features must be greater than 0", what);
+  if (!ind && o->individualized)
+    return fail(TM_E_ARG, "%s: individualized machines are created by the _ind entry points (feat_offsets)", what);
   if (o->state != 8) return fail(TM_E_UNSUPPORTED, "%s: state bits other than 8 are not built in this engine", what);
   return TM_OK;
 }
@@ -180,10 +206,45 @@ int size_check(uint64_t classes, uint64_t clauses, uint64_t input_chunks, uint64
 int alloc_device(tm_machine *m)
 {
   int rc;
-  if ((rc = m->actions.ensure(m->nclauses() * m->wd * 4))) return rc;
-  if ((rc = m->counts.ensure(m->nclauses() * (m->state_bits - 1) * m->wd * 4))) return rc;
-  HIP_TRY(tm_launch_zero(m->actions.p, m->nclauses() * m->wd * 4, m->stream));
+  const uint64_t aw = m->action_words();
+  if ((rc = m->actions.ensure(aw * 4))) return rc;
+  if ((rc = m->counts.ensure(aw * (m->state_bits - 1) * 4))) return rc;
+  HIP_TRY(tm_launch_zero(m->actions.p, aw * 4, m->stream));
   m->has_state = true;
+  return TM_OK;
+}
+
+// Per-class sizes and offsets of an individualized machine from its feature
+// sizes and clause count (capi.c:576-593): class h takes clauses x
+// ceil(2 k_h / 8) bytes of actions, after the classes before it.  On the
+// device each row is padded to whole u32 words.
+int ind_layout(tm_machine *m, const char *what)
+{
+  uint64_t aw = 0, hb = 0;
+  uint32_t wmax = 1;
+  for (size_t h = 0; h < m->ind.size(); h++) {
+    tm_ind_class &c = m->ind[h];
+    if (c.features == 0) return fail(TM_E_ARG, "%s: class %zu has an empty feature set", what, h);
+    if (c.features > 0x7FFFFFFFull) return fail(TM_E_ARG, "%s: class %zu has too many features", what, h);
+    c.input_bits = (uint32_t)(2 * c.features);
+    c.input_chunks = (c.input_bits + 7) / 8;
+    const uint32_t tail_bits = c.input_bits & 7;
+    c.tail_mask = tail_bits ? (uint8_t)((1u << tail_bits) - 1) : 0xFF;
+    c.wd = (c.input_chunks + 3) / 4;
+    int rc = size_check(1, m->clauses, c.input_chunks, m->state_bits);
+    if (rc) return rc;
+    c.aoff = aw;
+    c.coff = aw * (m->state_bits - 1);
+    c.hoff = hb;
+    aw += (uint64_t) m->clauses * c.wd;
+    hb += (uint64_t) m->clauses * c.input_chunks;
+    wmax = std::max(wmax, c.wd);
+  }
+  if (hb * (m->state_bits - 1) > 0xFFFFFFFFull)
+    return fail(TM_E_ARG, "%s: machine too large for the persisted 32-bit size fields", what);
+  m->action_chunks = (uint32_t) hb;
+  m->state_chunks = (uint32_t)(hb * (m->state_bits - 1));
+  m->wd = wmax;
   return TM_OK;
 }
 
@@ -243,6 +304,87 @@ int create_common(const tm_options *o, int type, tm_machine **out, hipStream_t s
   return TM_OK;
 }
 
+// tk_tsetlin_init_classifier_ind / _init_encoder_ind, capi.c:528-626 (create
+// dispatch capi.c:631-645, :668-682; reusable is read after init, :659).
+// Class h has feat_offsets[h + 1] - feat_offsets[h] features.  The machine-wide
+// features, input_bits, input_chunks and tail_mask stay 0 (the reference
+// never sets them for an individualized machine).
+int create_ind(const tm_options *o, const int64_t *feat_offsets, uint32_t n_offsets, int type, tm_machine **out)
+{
+  const char *what = type == TM_CLASSIFIER ? "create classifier ind" : "create encoder ind";
+  if (!out) return fail(TM_E_ARG, "%s: out is NULL", what);
+  *out = nullptr;
+  int rc = validate_opts(o, what, true);
+  if (rc) return rc;
+  if (!feat_offsets || (uint64_t) n_offsets != (uint64_t) o->classes + 1)
+    return fail(TM_E_ARG, "%s: feat_offsets must have classes + 1 elements", what);
+  for (uint32_t h = 0; h < o->classes; h++)
+    if (feat_offsets[h + 1] < feat_offsets[h]) return fail(TM_E_ARG, "%s: feat_offsets must not decrease", what);
+  tm_machine *m = new tm_machine();
+  m->type = type;
+  m->individualized = true;
+  m->negative = o->negative < 0 ? 1.0 / (double) o->classes : o->negative;
+  m->classes = o->classes;
+  m->class_chunks = (o->classes + 7) / 8;
+  m->clauses = ((o->clauses + 7) / 8) * 8;
+  m->clause_chunks = m->clauses / 8;
+  m->clause_tolerance = o->clause_tolerance;
+  m->clause_maximum = o->clause_maximum;
+  m->target = derive_target(o->target, m->clauses, o->clause_tolerance);
+  m->state_bits = o->state;
+  m->include_bits = o->include_bits ? o->include_bits : 1;
+  m->specificity = o->specificity;
+  m->reusable = o->reusable != 0;
+  m->seed = o->seed;
+  m->features = 0;
+  m->input_bits = 0;
+  m->input_chunks = 0;
+  m->tail_mask = 0;
+  m->ind.resize(m->classes);
+  for (uint32_t h = 0; h < m->classes; h++) m->ind[h].features = (uint64_t)(feat_offsets[h + 1] - feat_offsets[h]);
+  if ((rc = ind_layout(m, what))) { delete m; return rc; }
+  if ((rc = init_runtime(m)) || (rc = alloc_device(m))) { tm_destroy(m); return rc; }
+  *out = m;
+  return TM_OK;
+}
+
+// Bytes of an individualized problems buffer that a call reads: class h's
+// rows are the n rows of input_chunks_h bytes from byte dim_offsets[h]
+// (capi.c:1348-1350 train, :1022-1024 predict).
+int ind_extent(const tm_machine *m, uint32_t n, const int64_t *dim_offsets, size_t bytes, const char *what,
+               size_t *extent)
+{
+  if (!dim_offsets) return fail(TM_E_ARG, "%s: dim_offsets must not be NULL", what);
+  uint64_t e = 0;
+  for (size_t h = 0; h < m->ind.size(); h++) {
+    if (dim_offsets[h] < 0) return fail(TM_E_ARG, "%s: dim_offsets[%zu] is negative", what, h);
+    e = std::max<uint64_t>(e, (uint64_t) dim_offsets[h] + (uint64_t) n * m->ind[h].input_chunks);
+  }
+  if (e > bytes)
+    return fail(TM_E_SIZE, "%s: problems hold %zu bytes, dim_offsets and n need %llu", what, bytes,
+                (unsigned long long) e);
+  *extent = (size_t) e;
+  return TM_OK;
+}
+
+// Stage an individualized problems buffer (host) as per-class u32 rows:
+// class h's n rows start at word woff[h], strides[h] words apart.
+int stage_rows_ind(tm_machine *m, const uint8_t *rows, uint32_t n, size_t extent, const int64_t *dim_offsets,
+                   const std::vector<uint32_t> &strides, dbuf &bytes_buf, dbuf &words_buf, std::vector<uint64_t> &woff)
+{
+  int rc;
+  if ((rc = bytes_buf.ensure(extent))) return rc;
+  HIP_TRY(hipMemcpyAsync(bytes_buf.p, rows, extent, hipMemcpyHostToDevice, m->stream));
+  woff.assign(m->ind.size(), 0);
+  uint64_t total = 0;
+  for (size_t h = 0; h < m->ind.size(); h++) { woff[h] = total; total += (uint64_t) n * strides[h]; }
+  if ((rc = words_buf.ensure(total * 4))) return rc;
+  for (size_t h = 0; h < m->ind.size(); h++)
+    HIP_TRY(tm_launch_repack(bytes_buf.as<uint8_t>() + dim_offsets[h], n, m->ind[h].input_chunks, strides[h],
+                             words_buf.as<uint32_t>() + woff[h], m->stream));
+  return TM_OK;
+}
+
 // Stage host or device rows (n x input_chunks bytes) as u32 words [n][stride]
 // (stride >= wd; words past the row are zero).  Rows are used in place when
 // the caller's bytes already have that layout and 16-byte alignment.
@@ -298,11 +440,60 @@ int prepare_predict(tm_machine *m)
 {
   if (m->predict_ready) return TM_OK;
   int rc;
-  if ((rc = m->actT.ensure(m->nclauses() * m->wd * 4))) return rc;
+  if ((rc = m->actT.ensure(m->action_words() * 4))) return rc;
   if ((rc = m->lit.ensure(m->nclauses() * 4))) return rc;
-  HIP_TRY(tm_launch_prepare_predict(m->actions.as<uint32_t>(), m->actT.as<uint32_t>(), m->lit.as<uint32_t>(),
-                                    (uint32_t) m->nclauses(), m->wd, m->last_eval_mask(), m->stream));
+  if (m->individualized) {
+    for (size_t h = 0; h < m->ind.size(); h++) {
+      const tm_ind_class &c = m->ind[h];
+      HIP_TRY(tm_launch_prepare_predict(m->actions.as<uint32_t>() + c.aoff, m->actT.as<uint32_t>() + c.aoff,
+                                        m->lit.as<uint32_t>() + h * m->clauses, m->clauses, c.wd,
+                                        eval_mask(c.wd, c.input_bits), m->stream));
+    }
+  } else {
+    HIP_TRY(tm_launch_prepare_predict(m->actions.as<uint32_t>(), m->actT.as<uint32_t>(), m->lit.as<uint32_t>(),
+                                      (uint32_t) m->nclauses(), m->wd, m->last_eval_mask(), m->stream));
+  }
   m->predict_ready = true;
+  return TM_OK;
+}
+
+// Class sums of an individualized machine (capi.c:1010-1029, :1057-1073): one
+// predict launch per class over that class's own rows, each adding into its
+// column of the [n][classes] sums.
+int predict_sums_ind(tm_machine *m, const uint8_t *rows, size_t bytes, const int64_t *dim_offsets, uint32_t n)
+{
+  int rc;
+  size_t extent;
+  if ((rc = ind_extent(m, n, dim_offsets, bytes, "predict", &extent))) return rc;
+  if ((rc = use_device(m))) return rc;
+  if ((rc = prepare_predict(m))) return rc;
+  std::vector<uint32_t> strides(m->classes);
+  for (uint32_t h = 0; h < m->classes; h++) strides[h] = m->ind[h].wd;
+  std::vector<uint64_t> woff;
+  if ((rc = stage_rows_ind(m, rows, n, extent, dim_offsets, strides, m->p_rows, m->p_words, woff))) return rc;
+  if ((rc = m->p_sums.ensure((size_t) n * m->classes * 4))) return rc;
+  HIP_TRY(tm_launch_zero(m->p_sums.p, (size_t) n * m->classes * 4, m->stream));
+  HIP_TRY(hipEventRecord(m->ev0, m->stream));
+  for (uint32_t h = 0; h < m->classes; h++) {
+    const tm_ind_class &c = m->ind[h];
+    tm_predict_launch L;
+    memset(&L, 0, sizeof L);
+    L.actT = m->actT.as<uint32_t>() + c.aoff;
+    L.lit = m->lit.as<uint32_t>() + (size_t) h * m->clauses;
+    L.rows = m->p_words.as<uint32_t>() + woff[h];
+    L.sums = m->p_sums.as<int32_t>();
+    L.n = n;
+    L.wd = c.wd;
+    L.clauses_total = m->clauses;
+    L.clauses = m->clauses;
+    L.classes = 1;
+    L.tolerance = m->clause_tolerance;
+    L.last_eval = eval_mask(c.wd, c.input_bits);
+    L.sums_stride = m->classes;
+    L.class_offset = h;
+    HIP_TRY(tm_launch_predict(L, m->stream));
+  }
+  HIP_TRY(hipEventRecord(m->ev1, m->stream));
   return TM_OK;
 }
 
@@ -316,6 +507,7 @@ int predict_sums(tm_machine *m, const uint8_t *rows, uint32_t n, bool dev)
   if ((rc = m->p_sums.ensure((size_t) n * m->classes * 4))) return rc;
   HIP_TRY(tm_launch_zero(m->p_sums.p, (size_t) n * m->classes * 4, m->stream));
   tm_predict_launch L;
+  memset(&L, 0, sizeof L);
   L.actT = m->actT.as<uint32_t>();
   L.lit = m->lit.as<uint32_t>();
   L.rows = words;
@@ -351,14 +543,20 @@ int predict_enqueue_dev(tm_machine *m, const uint8_t *rows, uint32_t n, int64_t 
   return TM_OK;
 }
 
-int predict_impl(tm_machine *m, const uint8_t *rows, uint32_t n, void *out, bool dev, int want_type)
+// Individualized calls pass dim_offsets (never NULL) and the byte size of rows.
+int predict_impl(tm_machine *m, const uint8_t *rows, uint32_t n, void *out, bool dev, int want_type,
+                 bool ind_call = false, const int64_t *dim_offsets = nullptr, size_t rows_bytes = 0)
 {
   int rc;
   if ((rc = check_handle(m))) return rc;
   if (m->type != want_type) return fail(TM_E_ARG, "predict: machine is not a%s", want_type == TM_CLASSIFIER ? " classifier" : "n encoder");
+  if (m->individualized && !ind_call)
+    return fail(TM_E_ARG, "predict: individualized machine: predict with dim_offsets (the _ind entry points)");
+  if (!m->individualized && ind_call)
+    return fail(TM_E_ARG, "predict: dim_offsets given to a machine that is not individualized");
   if (n == 0) return TM_OK;
   if (!rows || !out) return fail(TM_E_ARG, "predict: problems and out must not be NULL");
-  if ((rc = predict_sums(m, rows, n, dev))) return rc;
+  if ((rc = ind_call ? predict_sums_ind(m, rows, rows_bytes, dim_offsets, n) : predict_sums(m, rows, n, dev))) return rc;
   const size_t out_bytes = want_type == TM_CLASSIFIER ? (size_t) n * 8 : (size_t) n * m->class_chunks;
   void *dout = out;
   if (!dev) {
@@ -446,13 +644,20 @@ int build_queue(tm_machine *m, tm_train_launch &L, const uint32_t ek[4], uint32_
   return TM_OK;
 }
 
-// tk_tsetlin_train_classifier / _encoder, capi.c:1112-1282.
+// tk_tsetlin_train_classifier / _encoder, capi.c:1112-1282; individualized
+// machines (ind_call, with dim_offsets and the byte size of rows):
+// tk_tsetlin_train_classifier_ind / _encoder_ind, capi.c:1284-1482.
 int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *labels, const uint8_t *codes,
-               uint32_t iterations, tm_epoch_fn each, void *user, bool dev, int want_type)
+               uint32_t iterations, tm_epoch_fn each, void *user, bool dev, int want_type,
+               bool ind_call = false, const int64_t *dim_offsets = nullptr, size_t rows_bytes = 0)
 {
   int rc;
   if ((rc = check_handle(m))) return rc;
   if (m->type != want_type) return fail(TM_E_ARG, "train: machine is not a%s", want_type == TM_CLASSIFIER ? " classifier" : "n encoder");
+  if (m->individualized && !ind_call)
+    return fail(TM_E_ARG, "train: individualized machine: train with dim_offsets (the _ind entry points)");
+  if (!m->individualized && ind_call)
+    return fail(TM_E_ARG, "train: dim_offsets given to a machine that is not individualized");
   if (!m->has_state)
     return fail(TM_E_STATE, "can't train a model loaded without state (or released after training; create with reusable=true to train again)");
   if (n > 0 && !rows) return fail(TM_E_ARG, "train: problems must not be NULL");
@@ -464,9 +669,10 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
   memset(m->stats, 0, sizeof m->stats);
   m->predict_ready = false;
   if (iterations == 0 || n == 0) {
-    // setup still runs (capi.c:1136-1146) even when no sample is visited
-    HIP_TRY(tm_launch_setup(m->counts.as<uint32_t>(), m->actions.as<uint32_t>(), m->nclauses(),
-                            m->state_bits - 1, m->wd, m->stream));
+    // setup still runs (capi.c:1136-1146) even when no sample is visited; it
+    // fills every word alike, so the whole state goes in one call
+    HIP_TRY(tm_launch_setup(m->counts.as<uint32_t>(), m->actions.as<uint32_t>(), m->action_words(),
+                            m->state_bits - 1, 1, m->stream));
     HIP_TRY(hipStreamSynchronize(m->stream));
     for (uint32_t e = 0; e < iterations; e++) {
       if (!each) continue;
@@ -477,7 +683,20 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
   } else {
     const uint32_t *words;
     const uint32_t stride = train_row_stride(m);
-    if ((rc = stage_rows(m, rows, n, dev, stride, m->t_rows, m->t_words, &words))) return rc;
+    std::vector<uint32_t> strides;     // individualized: per-class row strides and word offsets
+    std::vector<uint64_t> woff;
+    if (ind_call) {
+      size_t extent;
+      if ((rc = ind_extent(m, n, dim_offsets, rows_bytes, "train", &extent))) return rc;
+      uint32_t lanes, wpl;
+      train_tile(m, &lanes, &wpl);
+      strides.resize(m->classes);
+      for (uint32_t h = 0; h < m->classes; h++) strides[h] = (m->ind[h].wd + wpl - 1) / wpl * wpl;
+      if ((rc = stage_rows_ind(m, rows, n, extent, dim_offsets, strides, m->t_rows, m->t_words, woff))) return rc;
+      words = m->t_words.as<uint32_t>();
+    } else if ((rc = stage_rows(m, rows, n, dev, stride, m->t_rows, m->t_words, &words))) {
+      return rc;
+    }
     const int64_t *dlabels = nullptr;
     const uint8_t *dcodes = nullptr;
     if (want_type == TM_CLASSIFIER) {
@@ -500,7 +719,7 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
     // classifier instead builds a step queue per epoch and launches once per
     // epoch: the queue takes the epoch-order scan, the negative-sampling
     // draws and the step draws out of the train kernel.
-    const bool queue = want_type == TM_CLASSIFIER && queue_enabled();
+    const bool queue = want_type == TM_CLASSIFIER && queue_enabled() && !m->individualized;
     const uint32_t group = (each || queue) ? 1u : 8u;
     if ((rc = m->t_perm.ensure((size_t) group * n * 4))) return rc;
     if ((rc = m->t_perm_scratch.ensure(tm_perm_scratch_bytes(n)))) return rc;
@@ -550,6 +769,38 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
     L.last_eval = m->last_eval_mask();
     L.all_ib1 = m->include_bits <= 1;
     train_tile(m, &L.lanes, &L.wpl);
+    // An individualized machine trains as a table of one-class machines, one
+    // per class, in one launch: class h's state, rows, width, masks and
+    // random-literal count are its own (capi.c:1441-1454, s at capi.c:344);
+    // its chunk c draws as chunk h * clause_chunks + c of the stream.
+    std::vector<tm_train_trial> T;
+    if (m->individualized) {
+      T.assign(m->classes, P);
+      std::vector<uint32_t> map;
+      map.reserve((size_t) 2 * m->classes * m->clause_chunks);
+      for (uint32_t h = 0; h < m->classes; h++) {
+        const tm_ind_class &c = m->ind[h];
+        tm_train_trial &Q = T[h];
+        Q.counts = m->counts.as<uint32_t>() + c.coff;
+        Q.actions = m->actions.as<uint32_t>() + c.aoff;
+        Q.s_draws = (uint32_t)((double)(2 * c.features) / m->specificity);
+        Q.rows = words + woff[h];
+        Q.row_stride = strides[h];
+        Q.wd = c.wd;
+        Q.input_bits = c.input_bits;
+        Q.last_live = live_mask(c.wd, c.input_chunks, c.tail_mask);
+        Q.last_eval = eval_mask(c.wd, c.input_bits);
+        Q.class_base = h;
+        Q.chunk_base = h * m->clause_chunks;
+        for (uint32_t ch = 0; ch < m->clause_chunks; ch++) { map.push_back(h); map.push_back(ch); }
+      }
+      if ((rc = m->t_trials.ensure(T.size() * sizeof(tm_train_trial))) || (rc = m->t_map.ensure(map.size() * 4)))
+        return rc;
+      HIP_TRY(hipMemcpyAsync(m->t_map.p, map.data(), map.size() * 4, hipMemcpyHostToDevice, m->stream));
+      HIP_TRY(hipStreamSynchronize(m->stream));
+      L.trials = m->t_trials.as<tm_train_trial>();
+      L.block_map = m->t_map.as<uint32_t>();
+    }
     std::vector<uint32_t> ek(4 * group);
     bool stop = false;
     for (uint32_t e0 = 0; e0 < iterations && !stop; e0 += group) {
@@ -562,6 +813,11 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
       HIP_TRY(tm_launch_perm(m->seed, e0, cnt, n, m->t_perm.as<uint32_t>(), m->t_perm_scratch.p,
                              m->t_perm_scratch.cap, m->stream));
       P.perm_epoch0 = e0;
+      if (!T.empty()) {
+        for (tm_train_trial &Q : T) Q.perm_epoch0 = e0;
+        HIP_TRY(hipMemcpyAsync(m->t_trials.p, T.data(), T.size() * sizeof(tm_train_trial), hipMemcpyHostToDevice,
+                               m->stream));
+      }
       L.epoch_first = e0;
       L.epochs = cnt;
       L.init = e0 == 0;
@@ -607,32 +863,49 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
 
 void put_u32(std::vector<uint8_t> &v, uint32_t x) { v.insert(v.end(), (uint8_t *) &x, (uint8_t *) &x + 4); }
 
+// The include plane as runs of clauses that share a row width: the whole
+// machine, or one run per class of an individualized machine.
+struct action_run {
+  uint64_t clauses, bytes, wd, aoff, hoff;
+};
+
+std::vector<action_run> action_runs(const tm_machine *m)
+{
+  std::vector<action_run> r;
+  if (!m->individualized) {
+    r.push_back({ m->nclauses(), m->input_chunks, m->wd, 0, 0 });
+  } else {
+    for (const tm_ind_class &c : m->ind) r.push_back({ m->clauses, c.input_chunks, c.wd, c.aoff, c.hoff });
+  }
+  return r;
+}
+
 int actions_to_host(tm_machine *m, uint8_t *dst)
 {
   // device [clause][wd] words -> reference layout [clause][input_chunks] bytes
-  const uint64_t nc = m->nclauses();
-  const size_t B = m->input_chunks;
-  if (B == (size_t) m->wd * 4) {
-    HIP_TRY(hipMemcpyAsync(dst, m->actions.p, nc * B, hipMemcpyDeviceToHost, m->stream));
+  const std::vector<action_run> runs = action_runs(m);
+  if (runs.size() == 1 && runs[0].bytes == runs[0].wd * 4) {
+    HIP_TRY(hipMemcpyAsync(dst, m->actions.p, runs[0].clauses * runs[0].bytes, hipMemcpyDeviceToHost, m->stream));
     HIP_TRY(hipStreamSynchronize(m->stream));
     return TM_OK;
   }
-  std::vector<uint32_t> tmp(nc * m->wd);
+  std::vector<uint32_t> tmp(m->action_words());
   HIP_TRY(hipMemcpyAsync(tmp.data(), m->actions.p, tmp.size() * 4, hipMemcpyDeviceToHost, m->stream));
   HIP_TRY(hipStreamSynchronize(m->stream));
-  for (uint64_t c = 0; c < nc; c++) memcpy(dst + c * B, &tmp[c * m->wd], B);
+  for (const action_run &r : runs)
+    for (uint64_t c = 0; c < r.clauses; c++) memcpy(dst + r.hoff + c * r.bytes, &tmp[r.aoff + c * r.wd], r.bytes);
   return TM_OK;
 }
 
 int actions_from_host(tm_machine *m, const uint8_t *src)
 {
-  const uint64_t nc = m->nclauses();
-  const size_t B = m->input_chunks;
-  if (B == (size_t) m->wd * 4) {
-    HIP_TRY(hipMemcpyAsync(m->actions.p, src, nc * B, hipMemcpyHostToDevice, m->stream));
+  const std::vector<action_run> runs = action_runs(m);
+  if (runs.size() == 1 && runs[0].bytes == runs[0].wd * 4) {
+    HIP_TRY(hipMemcpyAsync(m->actions.p, src, runs[0].clauses * runs[0].bytes, hipMemcpyHostToDevice, m->stream));
   } else {
-    std::vector<uint32_t> tmp(nc * m->wd, 0);
-    for (uint64_t c = 0; c < nc; c++) memcpy(&tmp[c * m->wd], src + c * B, B);
+    std::vector<uint32_t> tmp(m->action_words(), 0);
+    for (const action_run &r : runs)
+      for (uint64_t c = 0; c < r.clauses; c++) memcpy(&tmp[r.aoff + c * r.wd], src + r.hoff + c * r.bytes, r.bytes);
     HIP_TRY(hipMemcpyAsync(m->actions.p, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice, m->stream));
   }
   HIP_TRY(hipStreamSynchronize(m->stream));
@@ -666,6 +939,8 @@ int serialize(tm_machine *m, std::vector<uint8_t> &v)
   int rc = actions_to_host(m, v.data() + off);
   if (rc) return rc;
   v.push_back(m->individualized ? 1 : 0);
+  // capi.c:1528-1530: an individualized machine appends its feature sizes
+  for (const tm_ind_class &c : m->ind) v.insert(v.end(), (const uint8_t *) &c.features, (const uint8_t *) &c.features + 8);
   return TM_OK;
 }
 
@@ -744,6 +1019,117 @@ int tm_predict_encoder_dev(tm_machine *m, const uint8_t *problems, uint32_t n, u
   return predict_impl(m, problems, n, out, true, TM_ENCODER);
 }
 
+int tm_create_classifier_ind(const tm_options *opts, const int64_t *feat_offsets, uint32_t n_offsets, tm_machine **out)
+{
+  return create_ind(opts, feat_offsets, n_offsets, TM_CLASSIFIER, out);
+}
+
+int tm_create_encoder_ind(const tm_options *opts, const int64_t *feat_offsets, uint32_t n_offsets, tm_machine **out)
+{
+  return create_ind(opts, feat_offsets, n_offsets, TM_ENCODER, out);
+}
+
+int tm_train_classifier_ind(tm_machine *m, uint32_t n, const uint8_t *problems, size_t problems_bytes,
+                            const int64_t *solutions, const int64_t *dim_offsets, uint32_t iterations,
+                            tm_epoch_fn each, void *user)
+{
+  return train_impl(m, n, problems, solutions, nullptr, iterations, each, user, false, TM_CLASSIFIER, true,
+                    dim_offsets, problems_bytes);
+}
+
+int tm_train_encoder_ind(tm_machine *m, uint32_t n, const uint8_t *sentences, size_t sentences_bytes,
+                         const uint8_t *codes, const int64_t *dim_offsets, uint32_t iterations, tm_epoch_fn each,
+                         void *user)
+{
+  return train_impl(m, n, sentences, nullptr, codes, iterations, each, user, false, TM_ENCODER, true, dim_offsets,
+                    sentences_bytes);
+}
+
+int tm_predict_classifier_ind(tm_machine *m, const uint8_t *problems, size_t problems_bytes,
+                              const int64_t *dim_offsets, uint32_t n, int64_t *out)
+{
+  return predict_impl(m, problems, n, out, false, TM_CLASSIFIER, true, dim_offsets, problems_bytes);
+}
+
+int tm_predict_encoder_ind(tm_machine *m, const uint8_t *problems, size_t problems_bytes,
+                           const int64_t *dim_offsets, uint32_t n, uint8_t *out)
+{
+  return predict_impl(m, problems, n, out, false, TM_ENCODER, true, dim_offsets, problems_bytes);
+}
+
+// capi.c:752-900: keep the classes listed in `selected`, in that order, with
+// their actions (and counts, while the machine has them).
+int tm_restrict(tm_machine *m, const int64_t *selected, uint32_t n_selected)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (!m->individualized) return fail(TM_E_ARG, "restrict only supported for individualized TMs");
+  if (n_selected == 0) return fail(TM_E_ARG, "selected_dims cannot be empty");
+  if (!selected) return fail(TM_E_ARG, "restrict: selected_dims is NULL");
+  for (uint32_t i = 0; i < n_selected; i++) {
+    const int64_t dim = selected[i];
+    if (dim < 0 || (uint64_t) dim >= m->classes)
+      return fail(TM_E_ARG, "selected_dims[%d] = %d out of range [0, %d)", (int) i, (int) dim, (int) m->classes);
+    for (uint32_t j = i + 1; j < n_selected; j++)
+      if (selected[i] == selected[j]) return fail(TM_E_ARG, "duplicate dimension %d in selected_dims", (int) dim);
+  }
+  if ((rc = use_device(m))) return rc;
+  const std::vector<tm_ind_class> old = m->ind;
+  const uint32_t old_classes = m->classes;
+  auto undo = [&]() {
+    m->ind = old;
+    m->classes = old_classes;
+    ind_layout(m, "restrict");
+  };
+  m->ind.assign(n_selected, tm_ind_class());
+  for (uint32_t i = 0; i < n_selected; i++) m->ind[i].features = old[(size_t) selected[i]].features;
+  m->classes = n_selected;
+  if ((rc = ind_layout(m, "restrict"))) { undo(); return rc; }
+  const uint64_t M = m->state_bits - 1;
+  dbuf na, nc;
+  if ((rc = na.ensure(m->action_words() * 4)) || (m->has_state && (rc = nc.ensure(m->action_words() * M * 4)))) {
+    na.release();
+    nc.release();
+    undo();
+    return rc;
+  }
+  hipError_t e = hipSuccess;
+  for (uint32_t i = 0; i < n_selected && e == hipSuccess; i++) {
+    const tm_ind_class &src = old[(size_t) selected[i]], &dst = m->ind[i];
+    const uint64_t words = (uint64_t) m->clauses * dst.wd;
+    e = hipMemcpyAsync(na.as<uint32_t>() + dst.aoff, m->actions.as<uint32_t>() + src.aoff, words * 4,
+                       hipMemcpyDeviceToDevice, m->stream);
+    if (e == hipSuccess && m->has_state)
+      e = hipMemcpyAsync(nc.as<uint32_t>() + dst.coff, m->counts.as<uint32_t>() + src.coff, words * M * 4,
+                         hipMemcpyDeviceToDevice, m->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e != hipSuccess) {
+    na.release();
+    nc.release();
+    undo();
+    return fail(TM_E_DEVICE, "restrict: device copy failed: %s", hipGetErrorString(e));
+  }
+  m->actions.release();
+  m->actions = na;
+  if (m->has_state) {
+    m->counts.release();
+    m->counts = nc;
+  }
+  m->class_chunks = (m->classes + 7) / 8;
+  m->predict_ready = false;
+  return TM_OK;
+}
+
+int tm_get_feature_sizes(const tm_machine *m, uint64_t *out, uint32_t cap)
+{
+  if (!m) return fail(TM_E_ARG, "machine is NULL");
+  if (!m->individualized) return fail(TM_E_ARG, "feature sizes: machine is not individualized");
+  if (!out || cap < m->classes) return fail(TM_E_SIZE, "feature sizes: room for %u, need %u", cap, m->classes);
+  for (uint32_t h = 0; h < m->classes; h++) out[h] = m->ind[h].features;
+  return TM_OK;
+}
+
 int tm_persist(tm_machine *m, uint8_t *buf, size_t cap, size_t *len)
 {
   int rc;
@@ -798,22 +1184,43 @@ int tm_load(const uint8_t *data, size_t len, tm_machine **out)
   m->tail_mask = tail;
   m->class_chunks = (m->classes + 7) / 8;
   m->negative = m->classes ? 1.0 / (double) m->classes : 0;   // not persisted (capi.c:1509-1531)
-  if (!m->classes || !m->clauses || m->clauses % 8 || m->clause_chunks != m->clauses / 8 ||
-      m->input_bits == 0 || m->input_chunks != (m->input_bits + 7) / 8 ||
-      (uint64_t) m->action_chunks != (uint64_t) m->classes * m->clauses * m->input_chunks) {
+  if (!m->classes || !m->clauses || m->clauses % 8 || m->clause_chunks != m->clauses / 8) {
     delete m;
     return fail(TM_E_FORMAT, "load: inconsistent header");
   }
-  int rc;
-  if ((rc = size_check(m->classes, m->clauses, m->input_chunks, m->state_bits < 2 ? 2 : m->state_bits))) { delete m; return rc; }
-  m->features = m->input_bits / 2;
   std::vector<uint8_t> acts(m->action_chunks);
   uint8_t ind = 0;
   if (!r.take(acts.data(), acts.size()) || !r.take(&ind, 1)) { delete m; return fail(TM_E_FORMAT, "load: truncated actions"); }
-  if (ind) { delete m; return fail(TM_E_UNSUPPORTED, "load: individualized machines are not built in this engine"); }
-  m->wd = (m->input_chunks + 3) / 4;
+  int rc;
+  if (ind) {
+    // capi.c:1698-1725: the feature sizes follow; the per-class layout is
+    // derived from them
+    m->individualized = true;
+    if ((uint64_t) m->classes * 8 > len - r.off) { delete m; return fail(TM_E_FORMAT, "load: truncated feature sizes"); }
+    m->ind.resize(m->classes);
+    for (uint32_t h = 0; h < m->classes; h++)
+      if (!r.take(&m->ind[h].features, 8)) { delete m; return fail(TM_E_FORMAT, "load: truncated feature sizes"); }
+    const uint32_t persisted_actions = m->action_chunks;
+    if (m->state_bits < 2) { delete m; return fail(TM_E_FORMAT, "load: inconsistent header"); }
+    if ((rc = ind_layout(m, "load"))) {
+      char msg[sizeof g_err];
+      snprintf(msg, sizeof msg, "%s", g_err);
+      delete m;
+      return rc == TM_E_ARG ? fail(TM_E_FORMAT, "%s", msg) : rc;
+    }
+    if (m->action_chunks != persisted_actions) { delete m; return fail(TM_E_FORMAT, "load: inconsistent header"); }
+  } else {
+    if (m->input_bits == 0 || m->input_chunks != (m->input_bits + 7) / 8 ||
+        (uint64_t) m->action_chunks != (uint64_t) m->classes * m->clauses * m->input_chunks) {
+      delete m;
+      return fail(TM_E_FORMAT, "load: inconsistent header");
+    }
+    if ((rc = size_check(m->classes, m->clauses, m->input_chunks, m->state_bits < 2 ? 2 : m->state_bits))) { delete m; return rc; }
+    m->features = m->input_bits / 2;
+    m->wd = (m->input_chunks + 3) / 4;
+  }
   if ((rc = init_runtime(m))) { tm_destroy(m); return rc; }
-  if ((rc = m->actions.ensure(m->nclauses() * m->wd * 4))) { tm_destroy(m); return rc; }
+  if ((rc = m->actions.ensure(m->action_words() * 4))) { tm_destroy(m); return rc; }
   if ((rc = actions_from_host(m, acts.data()))) { tm_destroy(m); return rc; }
   m->has_state = false;   // capi.c:1696: state = NULL after load
   m->trained = true;
@@ -869,14 +1276,26 @@ int tm_reconfigure(tm_machine *m, const tm_options *o)
   if (!(o->specificity > 0)) return fail(TM_E_ARG, "reconfigure: specificity must be a positive number");
   if ((rc = use_device(m))) return rc;
   const uint32_t nclauses = ((o->clauses + 7) / 8) * 8;
-  if ((rc = size_check(m->classes, nclauses, m->input_chunks, m->state_bits))) return rc;
-  m->clauses = nclauses;
+  if (m->individualized) {
+    // capi.c:1609-1615, :1651-1663: the per-class layout follows the new
+    // clause count
+    const uint32_t old = m->clauses;
+    m->clauses = nclauses;
+    if ((rc = ind_layout(m, "reconfigure"))) {
+      m->clauses = old;
+      ind_layout(m, "reconfigure");
+      return rc;
+    }
+  } else {
+    if ((rc = size_check(m->classes, nclauses, m->input_chunks, m->state_bits))) return rc;
+    m->clauses = nclauses;
+    m->action_chunks = m->classes * nclauses * m->input_chunks;
+    m->state_chunks = m->classes * nclauses * (m->state_bits - 1) * m->input_chunks;
+  }
   m->clause_chunks = nclauses / 8;
   m->clause_tolerance = o->clause_tolerance;
   m->clause_maximum = o->clause_maximum;
   m->include_bits = o->include_bits ? o->include_bits : 1;
-  m->action_chunks = m->classes * nclauses * m->input_chunks;
-  m->state_chunks = m->classes * nclauses * (m->state_bits - 1) * m->input_chunks;
   m->specificity = o->specificity;
   m->target = derive_target(o->target, nclauses, o->clause_tolerance);
   m->seed = o->seed;
@@ -899,7 +1318,7 @@ void tm_destroy(tm_machine *m)
   dbuf *bufs[] = { &m->counts, &m->actions, &m->actT, &m->lit, &m->t_rows, &m->t_words, &m->t_labels,
                    &m->t_codes, &m->t_perm, &m->t_perm_scratch, &m->t_ekeys, &m->p_rows, &m->p_words,
                    &m->p_sums, &m->p_out, &m->t_stats, &m->q_counts, &m->q_offsets, &m->q_scratch,
-                   &m->q_id, &m->q_draw };
+                   &m->q_id, &m->q_draw, &m->t_trials, &m->t_map };
   for (dbuf *b : bufs) b->release();
   if (m->ev0) hipEventDestroy(m->ev0);
   if (m->ev1) hipEventDestroy(m->ev1);
@@ -920,6 +1339,7 @@ int tm_get_info(const tm_machine *m, tm_info *o)
   o->action_chunks = m->action_chunks; o->class_chunks = m->class_chunks;
   o->specificity = m->specificity; o->negative = m->negative; o->tail_mask = m->tail_mask;
   o->reusable = m->reusable; o->has_state = m->has_state; o->trained = m->trained;
+  o->individualized = m->individualized;
   return TM_OK;
 }
 

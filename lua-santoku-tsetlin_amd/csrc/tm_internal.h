@@ -83,8 +83,10 @@ struct tm_predict_launch {
   const uint32_t *actT;    // [Wd][clauses_total] transposed include plane
   const uint32_t *lit;     // [clauses_total] literal counts
   const uint32_t *rows;    // [n][Wd]
-  int32_t *sums;           // [n][classes]
+  int32_t *sums;           // [n][sums_stride]; class k of this launch adds into column class_offset + k
   uint32_t n, wd, clauses_total, clauses, classes, tolerance, last_eval;
+  uint32_t sums_stride;    // 0: classes
+  uint32_t class_offset;
   uint32_t pass_tiles;     // set by the launcher: 64-clause tiles per L2-resident pass
 };
 

@@ -690,7 +690,7 @@ __global__ __launch_bounds__(64 * PRED_WAVES) void predict_kernel(tm_predict_lau
       int32_t t = (my_cls == k) ? v : 0;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-      if (lane == 0 && s < L.n) atomicAdd(&L.sums[(size_t) s * L.classes + k], t);
+      if (lane == 0 && s < L.n) atomicAdd(&L.sums[(size_t) s * L.sums_stride + L.class_offset + k], t);
     }
   }
 }
@@ -1002,13 +1002,15 @@ int tm_launch_predict(const tm_predict_launch &L, hipStream_t st)
   if (nct > max_blocks) return (int) hipErrorInvalidValue;
   const uint64_t st_per_launch = max_blocks / nct;
   tm_predict_launch P = L;
+  P.sums_stride = L.sums_stride ? L.sums_stride : L.classes;
+  if (P.class_offset + L.classes > P.sums_stride) return (int) hipErrorInvalidValue;
   // clause tiles per pass: about 2 MiB of include plane
   const uint64_t tile_bytes = 64ull * L.wd * 4;
   P.pass_tiles = (uint32_t) std::max<uint64_t>(1, std::min<uint64_t>(nct, (2ull << 20) / tile_bytes));
   for (uint64_t s0 = 0; s0 < L.n; s0 += st_per_launch * spt) {
     const uint64_t n = std::min<uint64_t>(L.n - s0, st_per_launch * spt);
     P.rows = L.rows + s0 * L.wd;
-    P.sums = L.sums + s0 * L.classes;
+    P.sums = L.sums + s0 * P.sums_stride;
     P.n = (uint32_t) n;
     const uint64_t nst = (n + spt - 1) / spt;
     hipLaunchKernelGGL(predict_kernel, dim3((uint32_t)(nst * nct)), dim3(64 * PRED_WAVES), 0, st, P);

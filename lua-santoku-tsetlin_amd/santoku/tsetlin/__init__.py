@@ -88,7 +88,18 @@ class Machine:
         _check(self._lib.tm_get_info(self._handle(), ctypes.byref(i)))
         return {f: getattr(i, f) for f, _ in _c.Info._fields_}
 
-    # capi.c:1484-1507 (dispatch), :1112-1191 (classifier), :1193-1282 (encoder)
+    def individualized(self):
+        return bool(self.info()["individualized"])
+
+    def feature_sizes(self):
+        """Per-class feature counts k_h of an individualized machine."""
+        k = self.info()["classes"]
+        out = (ctypes.c_uint64 * k)()
+        _check(self._lib.tm_get_feature_sizes(self._handle(), out, k))
+        return [int(v) for v in out]
+
+    # capi.c:1484-1507 (dispatch), :1112-1191 (classifier), :1193-1282 (encoder),
+    # :1284-1482 (individualized, with dim_offsets)
     def train(self, opts=None, **kw):
         o = dict(opts or {}, **kw)
         n = int(o["samples"])
@@ -108,6 +119,16 @@ class Machine:
             return 0 if r is False else 1
 
         cb = _c.EPOCH_FN(on_epoch) if each is not None else _c.EPOCH_FN()
+        if self.individualized():
+            if o.get("dim_offsets") is None:
+                raise TsetlinError("train: dim_offsets is required for an individualized machine")
+            ra, rp = _host_buffer(rows)
+            ta, tp = _host_buffer(tgt, np.uint8 if enc else np.int64)
+            da, dp = _host_buffer(o["dim_offsets"], np.int64)
+            keep += [ra, ta, da]
+            fn = self._lib.tm_train_encoder_ind if enc else self._lib.tm_train_classifier_ind
+            _check(fn(self._handle(), n, rp, ra.nbytes, tp, dp, iters, cb, None))
+            return
         if _is_device_tensor(rows):
             rp, tp = rows.data_ptr(), tgt.data_ptr()
             fn = self._lib.tm_train_encoder_dev if enc else self._lib.tm_train_classifier_dev
@@ -118,9 +139,14 @@ class Machine:
             fn = self._lib.tm_train_encoder if enc else self._lib.tm_train_classifier
         _check(fn(self._handle(), n, rp, tp, iters, cb, None))
 
-    # capi.c:1088-1110, :903-948, :950-995
-    def predict(self, problems, n, threads=None):
-        n = int(n)
+    # capi.c:1088-1110, :903-948, :950-995; an individualized machine takes
+    # predict(problems, dim_offsets, n[, threads]) (capi.c:997-1086)
+    def predict(self, problems, *args):
+        if self.individualized():
+            if len(args) < 2:
+                raise TsetlinError("predict: an individualized machine takes (problems, dim_offsets, n)")
+            return self._predict_ind(problems, args[0], int(args[1]))
+        n = int(args[0])
         enc = self.type() == "encoder"
         if _is_device_tensor(problems):
             import torch
@@ -141,6 +167,22 @@ class Machine:
             out = np.zeros(n, np.int64)
             _check(self._lib.tm_predict_classifier(self._handle(), pp, n, out.ctypes.data))
         return out
+
+    def _predict_ind(self, problems, dim_offsets, n):
+        pa, pp = _host_buffer(problems)
+        da, dp = _host_buffer(dim_offsets, np.int64)
+        if self.type() == "encoder":
+            out = np.zeros((n, (self.info()["classes"] + 7) // 8), np.uint8)
+            _check(self._lib.tm_predict_encoder_ind(self._handle(), pp, pa.nbytes, dp, n, out.ctypes.data))
+        else:
+            out = np.zeros(n, np.int64)
+            _check(self._lib.tm_predict_classifier_ind(self._handle(), pp, pa.nbytes, dp, n, out.ctypes.data))
+        return out
+
+    # capi.c:752-900
+    def restrict(self, selected_dims):
+        a, p = _host_buffer(selected_dims, np.int64)
+        _check(self._lib.tm_restrict(self._handle(), p, a.size))
 
     # capi.c:1538-1562: returns the bytes when path is None
     def persist(self, path=None, persist_state=None):
@@ -216,6 +258,14 @@ def create(type_, opts=None, **kw):
     o = dict(opts or {}, **kw)
     lib = _c.load_library()
     h = ctypes.c_void_p()
+    if type_ in ("classifier", "encoder") and o.get("individualized"):
+        # capi.c:631-645, :668-682: the feature sets come from feat_offsets
+        if o.get("feat_offsets") is None:
+            raise TsetlinError("create: feat_offsets is required for an individualized machine")
+        fa, fp = _host_buffer(o["feat_offsets"], np.int64)
+        fn = lib.tm_create_classifier_ind if type_ == "classifier" else lib.tm_create_encoder_ind
+        _check(fn(ctypes.byref(_options(o)), fp, fa.size, ctypes.byref(h)))
+        return Machine(h.value)
     if type_ == "classifier":
         _check(lib.tm_create_classifier(ctypes.byref(_options(o)), ctypes.byref(h)))
     elif type_ == "encoder":

@@ -25,6 +25,8 @@ EXPORTS = [
     "tm_persist_file", "tm_load", "tm_load_file", "tm_checkpoint", "tm_restore", "tm_reconfigure",
     "tm_type", "tm_destroy", "tm_get_info", "tm_last_error", "tm_last_timing", "tm_last_train_stats",
     "tm_last_queue_timing",
+    "tm_create_classifier_ind", "tm_create_encoder_ind", "tm_train_classifier_ind", "tm_train_encoder_ind",
+    "tm_predict_classifier_ind", "tm_predict_encoder_ind", "tm_restrict", "tm_get_feature_sizes",
     "tm_batch_create", "tm_batch_machine", "tm_batch_train_classifier", "tm_batch_train_classifier_dev",
     "tm_batch_accuracy_dev", "tm_batch_tuning", "tm_batch_timing", "tm_batch_destroy",
 ]
@@ -55,7 +57,7 @@ class Info(ctypes.Structure):
         "action_chunks", "class_chunks")] + [
         ("specificity", ctypes.c_double), ("negative", ctypes.c_double),
         ("tail_mask", ctypes.c_uint8), ("reusable", ctypes.c_uint8), ("has_state", ctypes.c_uint8),
-        ("trained", ctypes.c_uint8)]
+        ("trained", ctypes.c_uint8), ("individualized", ctypes.c_uint8)]
 
 
 EPOCH_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32)
@@ -103,6 +105,14 @@ def load_library(path=LIB_PATH):
         "tm_last_timing": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "tm_last_train_stats": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64)]),
         "tm_last_queue_timing": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double)]),
+        "tm_create_classifier_ind": (ctypes.c_int, [opt_p, i64p, u32, m_pp]),
+        "tm_create_encoder_ind": (ctypes.c_int, [opt_p, i64p, u32, m_pp]),
+        "tm_train_classifier_ind": (ctypes.c_int, [vp, u32, u8p, ctypes.c_size_t, i64p, i64p, u32, EPOCH_FN, vp]),
+        "tm_train_encoder_ind": (ctypes.c_int, [vp, u32, u8p, ctypes.c_size_t, u8p, i64p, u32, EPOCH_FN, vp]),
+        "tm_predict_classifier_ind": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, i64p, u32, i64p]),
+        "tm_predict_encoder_ind": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, i64p, u32, u8p]),
+        "tm_restrict": (ctypes.c_int, [vp, i64p, u32]),
+        "tm_get_feature_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64), u32]),
         "tm_batch_create": (ctypes.c_int, [opt_p, u32, m_pp]),
         "tm_batch_machine": (ctypes.c_int, [vp, u32, m_pp]),
         "tm_batch_train_classifier": (ctypes.c_int, [vp, u32, u8p, i64p, u32, BATCH_EPOCH_FN, vp]),
