@@ -26,12 +26,17 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["tm_kernels.hip", "tm_api.cpp", "tm_mlp.hip"]
+# (source, extra defines, object name); tm_train_states.hip is built once per
+# plane count other than 7 (state bits 2..16 except 8)
+SOURCES = [("tm_kernels.hip", [], "tm_kernels.o"), ("tm_api.cpp", [], "tm_api.o"), ("tm_mlp.hip", [], "tm_mlp.o")]
+SOURCES += [("tm_train_states.hip", ["-DTM_PLANES=%d" % m], "tm_train_planes_%d.o" % m)
+            for m in range(1, 16) if m != 7]
 
 
-def _compile(src):
-    obj = os.path.join(BUILD, src + ".o")
-    cmd = [HIPCC, *COMMON, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(job):
+    src, defs, name = job
+    obj = os.path.join(BUILD, name)
+    cmd = [HIPCC, *COMMON, *defs, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr[-6000:]))
@@ -40,7 +45,9 @@ def _compile(src):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(len(SOURCES)) as ex:
+    workers = max(1, min(len(SOURCES), os.cpu_count() or 1, 16))
+    # the longest compile first
+    with ThreadPoolExecutor(workers) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]

@@ -184,11 +184,14 @@ int validate_opts(const tm_options *o, const char *what, bool ind = false)
   if (!ind && !o->features) return fail(TM_E_ARG, "%s: features must be greater than 0", what);
   if (!ind && o->individualized)
     return fail(TM_E_ARG, "%s: individualized machines are created by the _ind entry points (feat_offsets)", what);
-  if (o->state != 8) return fail(TM_E_UNSUPPORTED, "%s: state bits other than 8 are not built in this engine", what);
+  if (o->state > TM_MAX_STATE_BITS)
+    return fail(TM_E_UNSUPPORTED, "%s: state bits above %d are not built in this engine", what, TM_MAX_STATE_BITS);
   return TM_OK;
 }
 
-int size_check(uint64_t classes, uint64_t clauses, uint64_t input_chunks, uint64_t state_bits)
+// Size limits of a machine; `train` adds the train tile's (a loaded machine
+// only predicts).
+int size_check(uint64_t classes, uint64_t clauses, uint64_t input_chunks, uint64_t state_bits, bool train = true)
 {
   uint64_t act = classes * clauses * input_chunks;
   uint64_t sta = act * (state_bits - 1);
@@ -197,9 +200,10 @@ int size_check(uint64_t classes, uint64_t clauses, uint64_t input_chunks, uint64
   if ((uint64_t) clauses * 64ull > 0x7FFFFFFFull)
     return fail(TM_E_ARG, "clauses per class too large for 32-bit class sums");
   uint32_t lanes, wpl;
-  if (tm_train_shape((uint32_t)((input_chunks + 3) / 4), &lanes, &wpl))
-    return fail(TM_E_UNSUPPORTED, "rows of %llu bytes exceed the largest train tile (4096 bytes)",
-                (unsigned long long) input_chunks);
+  if (train && tm_train_shape((uint32_t)((input_chunks + 3) / 4), (uint32_t) state_bits, &lanes, &wpl))
+    return fail(TM_E_UNSUPPORTED, "rows of %llu bytes exceed the largest train tile for %llu state bits (%u bytes)",
+                (unsigned long long) input_chunks, (unsigned long long) state_bits,
+                state_bits * 16 <= 128 ? 4096u : 2048u);
   return TM_OK;
 }
 
@@ -217,8 +221,9 @@ int alloc_device(tm_machine *m)
 // Per-class sizes and offsets of an individualized machine from its feature
 // sizes and clause count (capi.c:576-593): class h takes clauses x
 // ceil(2 k_h / 8) bytes of actions, after the classes before it.  On the
-// device each row is padded to whole u32 words.
-int ind_layout(tm_machine *m, const char *what)
+// device each row is padded to whole u32 words.  `train` adds the train
+// tile's limits (a loaded machine only predicts).
+int ind_layout(tm_machine *m, const char *what, bool train = true)
 {
   uint64_t aw = 0, hb = 0;
   uint32_t wmax = 1;
@@ -231,7 +236,7 @@ int ind_layout(tm_machine *m, const char *what)
     const uint32_t tail_bits = c.input_bits & 7;
     c.tail_mask = tail_bits ? (uint8_t)((1u << tail_bits) - 1) : 0xFF;
     c.wd = (c.input_chunks + 3) / 4;
-    int rc = size_check(1, m->clauses, c.input_chunks, m->state_bits);
+    int rc = size_check(1, m->clauses, c.input_chunks, m->state_bits, train);
     if (rc) return rc;
     c.aoff = aw;
     c.coff = aw * (m->state_bits - 1);
@@ -418,12 +423,12 @@ void train_tile(const tm_machine *m, uint32_t *lanes, uint32_t *wpl)
 {
   *lanes = 64;
   *wpl = 1;
-  tm_train_shape(m->wd, lanes, wpl);
+  tm_train_shape(m->wd, m->state_bits, lanes, wpl);
   const char *v = getenv("TSETLIN_AMD_TILE");
   unsigned g = 0, w = 0;
   if (v && sscanf(v, "%ux%u", &g, &w) == 2) {
     uint32_t shapes[16][2];
-    const int n = tm_train_shapes(m->wd, shapes, 16);
+    const int n = tm_train_shapes(m->wd, m->state_bits, shapes, 16);
     for (int i = 0; i < n; i++)
       if (shapes[i][0] == g && shapes[i][1] == w) { *lanes = g; *wpl = w; }
   }
@@ -719,7 +724,7 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
     // classifier instead builds a step queue per epoch and launches once per
     // epoch: the queue takes the epoch-order scan, the negative-sampling
     // draws and the step draws out of the train kernel.
-    const bool queue = want_type == TM_CLASSIFIER && queue_enabled() && !m->individualized;
+    const bool queue = want_type == TM_CLASSIFIER && queue_enabled() && !m->individualized && m->state_bits == 8;
     const uint32_t group = (each || queue) ? 1u : 8u;
     if ((rc = m->t_perm.ensure((size_t) group * n * 4))) return rc;
     if ((rc = m->t_perm_scratch.ensure(tm_perm_scratch_bytes(n)))) return rc;
@@ -1202,7 +1207,7 @@ int tm_load(const uint8_t *data, size_t len, tm_machine **out)
       if (!r.take(&m->ind[h].features, 8)) { delete m; return fail(TM_E_FORMAT, "load: truncated feature sizes"); }
     const uint32_t persisted_actions = m->action_chunks;
     if (m->state_bits < 2) { delete m; return fail(TM_E_FORMAT, "load: inconsistent header"); }
-    if ((rc = ind_layout(m, "load"))) {
+    if ((rc = ind_layout(m, "load", false))) {
       char msg[sizeof g_err];
       snprintf(msg, sizeof msg, "%s", g_err);
       delete m;
@@ -1215,7 +1220,7 @@ int tm_load(const uint8_t *data, size_t len, tm_machine **out)
       delete m;
       return fail(TM_E_FORMAT, "load: inconsistent header");
     }
-    if ((rc = size_check(m->classes, m->clauses, m->input_chunks, m->state_bits < 2 ? 2 : m->state_bits))) { delete m; return rc; }
+    if ((rc = size_check(m->classes, m->clauses, m->input_chunks, m->state_bits < 2 ? 2 : m->state_bits, false))) { delete m; return rc; }
     m->features = m->input_bits / 2;
     m->wd = (m->input_chunks + 3) / 4;
   }
@@ -1435,7 +1440,7 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   }
   // rows: a stride every candidate tile can read
   uint32_t shapes[16][2];
-  const int nshapes = tm_train_shapes(m0->wd, shapes, 16);
+  const int nshapes = tm_train_shapes(m0->wd, m0->state_bits, shapes, 16);
   if (nshapes <= 0) return fail(TM_E_UNSUPPORTED, "batch train: no train tile covers %u words", m0->wd);
   uint32_t maxw = 1;
   for (int i = 0; i < nshapes; i++) maxw = std::max(maxw, shapes[i][1]);

@@ -109,10 +109,14 @@ int tm_launch_argmax(const int32_t *sums, uint32_t n, uint32_t classes, int64_t 
 int tm_launch_encode(const int32_t *sums, uint32_t n, uint32_t classes, uint8_t *out, hipStream_t st);
 int tm_launch_repack(const uint8_t *src, uint32_t n, uint32_t bytes, uint32_t stride_words, uint32_t *dst, hipStream_t st);
 int tm_launch_zero(void *p, size_t bytes, hipStream_t st);
+// State bits the train kernels are built for: 2 .. TM_MAX_STATE_BITS.  A tile
+// is built for state bits b when b x words per lane <= 128 (the chunk state
+// of a lane in VGPRs).
+#define TM_MAX_STATE_BITS 16
 // Train kernel tile for a row of wd words: lanes per clause and words per lane.
-int tm_train_shape(uint32_t wd, uint32_t *lanes, uint32_t *words_per_lane);
+int tm_train_shape(uint32_t wd, uint32_t state_bits, uint32_t *lanes, uint32_t *words_per_lane);
 // Every instantiated tile that covers wd words (for the autotuner); returns the count.
-int tm_train_shapes(uint32_t wd, uint32_t (*out)[2], int cap);
+int tm_train_shapes(uint32_t wd, uint32_t state_bits, uint32_t (*out)[2], int cap);
 int tm_launch_epoch_advance(uint32_t *epoch_dev, hipStream_t st);
 // hits[t] += number of s < n with pred[s] == labels[s] (one counter per launch)
 int tm_launch_count_hits(const int64_t *pred, const int64_t *labels, uint32_t n, unsigned long long *hits,

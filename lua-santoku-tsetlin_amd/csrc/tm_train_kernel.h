@@ -1,0 +1,546 @@
+// tm_train_kernel.h -- the train kernel template and its device helpers.
+//
+// Included by tm_kernels.hip (8 state bits, every mode) and by
+// tm_train_states.hip (other state bits, one object per plane count).
+#pragma once
+#include "tm_internal.h"
+#include "tm_rng.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// helpers
+
+// Sum over the G consecutive lanes of a clause group, returned to every lane
+// of the group.  Full waves reduce with DPP row shifts and row broadcasts
+// (lane 63 ends with the total); half waves the same, with the broadcast
+// kept inside each half (lanes 31 and 63 end with the totals); smaller
+// groups use butterfly shuffles.
+template <int G>
+__device__ __forceinline__ uint32_t group_sum(uint32_t v)
+{
+  if (G == 64) {
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, true);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xF, 0xF, true);   // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xF, 0xF, true);   // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+  }
+  if (G == 32) {
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x118, 0xF, 0xF, true);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, true);   // row_bcast:15 into rows 1, 3
+    const uint32_t lo = __builtin_amdgcn_readlane(v, 31), hi = __builtin_amdgcn_readlane(v, 63);
+    return (threadIdx.x & 32) ? hi : lo;
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+// Workgroup barrier that orders LDS only: outstanding global loads (the
+// prefetched literal row) stay in flight across it.
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int G>
+__device__ __forceinline__ bool group_any(bool pred)
+{
+  uint64_t b = __ballot(pred);
+  if (G == 64) return b != 0;
+  uint32_t lane = threadIdx.x & 63;
+  uint32_t base = lane & ~(uint32_t)(G - 1);
+  uint64_t gm = (G == 64) ? ~0ull : ((1ull << G) - 1);
+  return ((b >> base) & gm) != 0;
+}
+
+// Workgroups are dealt to the XCDs round-robin (workgroup b runs on XCD b % 8),
+// each XCD with its own L2.  This bijection on [0, nb) gives XCD x a contiguous
+// range of logical indices, so neighbouring tiles (the clause tiles of one
+// predict pass) share an L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t nb)
+{
+  const uint32_t x = b % kXcds, per = nb / kXcds, extra = nb % kXcds;
+  return x * per + min(x, extra) + b / kXcds;
+}
+
+__device__ __forceinline__ uint32_t word_mask(uint32_t w, uint32_t wd, uint32_t last)
+{
+  return w + 1 < wd ? 0xFFFFFFFFu : (w + 1 == wd ? last : 0u);
+}
+
+// Per literal: every plane one (the +1 bound) / some plane 0..top one (not
+// at the -1 bound).
+template <int M, int WPL>
+__device__ __forceinline__ uint32_t all_ones(const uint32_t (&st)[M + 1][WPL], int k)
+{
+  uint32_t v = st[0][k];
+#pragma unroll
+  for (int p = 1; p <= M; p++) v &= st[p][k];
+  return v;
+}
+
+template <int M, int WPL, int TOP>
+__device__ __forceinline__ uint32_t any_one(const uint32_t (&st)[M + 1][WPL], int k)
+{
+  uint32_t v = st[0][k];
+#pragma unroll
+  for (int p = 1; p <= TOP; p++) v |= st[p][k];
+  return v;
+}
+
+// +1 (sgn all ones) or -1 (sgn zero) on the literals in m: the carry / borrow
+// ripple of automata.h:88-146 / :164-194 / :556-594.  The caller has already
+// dropped the literals at the bound (the saturation of those routines), so the
+// ripple never leaves plane M.  Branch-free, so groups of one wave may step in
+// different directions: two logic ops per plane.
+template <int M, int WPL>
+__device__ __forceinline__ void ripple(uint32_t (&st)[M + 1][WPL], int k, uint32_t m, uint32_t sgn)
+{
+  const uint32_t ns = ~sgn;
+#pragma unroll
+  for (int p = 0; p <= M; p++) {
+    const uint32_t s0 = st[p][k];
+    st[p][k] = s0 ^ m;
+    // m & (s0 ^ ns) as one three-input op (truth table over m, s0, ns:
+    // 0xF0 & (0xCC ^ 0xAA) = 0x60); left to itself the compiler spends an
+    // extra xor per plane here
+    if (p < M) m = __builtin_amdgcn_bitop3_b32(m, s0, ns, 0x60);
+  }
+}
+
+// include plane AND the top include_bits-1 count planes (automata.h:499-512).
+// IB1: include_bits == 1, where "included" is the include plane itself.
+template <int M, int WPL, bool IB1>
+__device__ __forceinline__ uint32_t included(const uint32_t (&st)[M + 1][WPL], int k, uint32_t ib)
+{
+  uint32_t v = st[M][k];
+  if (IB1) return v;
+#pragma unroll
+  for (int i = 1; i <= M; i++)
+    if ((uint32_t) i < ib) v &= st[M - i][k];
+  return v;
+}
+
+// Load WPL consecutive words of a literal row from word wbase.  Callers
+// clamp wbase into the row (the row stride is a multiple of WPL) so the load
+// is unconditional; words past the row's end are masked by every use.
+template <int WPL>
+__device__ __forceinline__ void load_words(const uint32_t *row, uint32_t wbase, uint32_t (&x)[WPL])
+{
+  if constexpr (WPL % 4 == 0) {
+#pragma unroll
+    for (int k = 0; k < WPL; k += 4) {
+      const uint4 v = *reinterpret_cast<const uint4 *>(row + wbase + k);
+      x[k] = v.x; x[k + 1] = v.y; x[k + 2] = v.z; x[k + 3] = v.w;
+    }
+  } else if constexpr (WPL == 2) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(row + wbase);
+    x[0] = v.x; x[1] = v.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < WPL; k++) x[k] = row[wbase + k];
+  }
+}
+
+// A workgroup-uniform struct copied word by word through readfirstlane: the
+// trial's parameters then live in scalar registers (loaded from either the
+// trial table or the kernel arguments, the compiler would keep them in VGPRs).
+template <typename T>
+__device__ __forceinline__ T to_sgpr(const T &v)
+{
+  static_assert(sizeof(T) % 4 == 0, "32-bit words");
+  T out;
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(&v);
+  uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+  for (int i = 0; i < (int) (sizeof(T) / 4); i++) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// training
+
+template <int WPL, int G, int M, bool IB1, bool QUEUE>
+__global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel(tm_train_launch L)
+{
+  constexpr int TW = G * WPL;                  // words per clause tile
+  const uint32_t tid = threadIdx.x;
+  // clause within the chunk; a full-wave group makes it wave-uniform, and
+  // saying so keeps every per-clause decision on the scalar unit
+  const uint32_t j = G == 64 ? __builtin_amdgcn_readfirstlane(tid / G) : tid / G;
+  const uint32_t g = tid % G;                  // lane within the clause group
+  const uint32_t lane = tid & 63;
+  // Chunks stay dealt round-robin over the XCDs: each XCD then holds a share
+  // of every class, which balances the per-class work (a chunk's sample count
+  // depends on its class).  Rows are not shared through L2 anyway: every chunk
+  // walks the epoch order at its own pace.
+  uint32_t trial = 0, ch = blockIdx.x;
+  if (L.block_map) {
+    trial = L.block_map[2 * blockIdx.x];
+    ch = L.block_map[2 * blockIdx.x + 1];
+  }
+  const tm_train_trial P = to_sgpr(L.trials ? L.trials[trial] : L.one);
+  if (!P.active) return;
+  const uint32_t epoch_first = L.epoch_dev ? *L.epoch_dev : L.epoch_first;
+  const bool init = L.epoch_dev ? epoch_first == 0 : L.init != 0;
+  const uint32_t cls = P.class_base + ch / P.clause_chunks;
+  const uint32_t clause = ch * 8 + j;                 // index into this trial's state
+  const uint32_t chr = P.chunk_base + ch;             // chunk and clause in the random stream
+  const uint32_t clause_r = chr * 8 + j;
+  const uint32_t wbase = g * WPL;
+  const uint32_t wd = P.wd;
+  uint32_t *counts = P.counts;
+  uint32_t *actions = P.actions;
+
+  __shared__ int32_t votes_lds[2][8];
+  __shared__ uint32_t hits[8][TW];
+
+  for (uint32_t i = tid; i < 8 * TW; i += 8 * G) (&hits[0][0])[i] = 0;
+
+  uint32_t st[M + 1][WPL];
+#pragma unroll
+  for (int k = 0; k < WPL; k++) {
+    uint32_t w = wbase + k;
+    if (init) {
+#pragma unroll
+      for (int p = 0; p < M; p++) st[p][k] = 0xFFFFFFFFu;
+      st[M][k] = 0;
+    } else if (w < wd) {
+#pragma unroll
+      for (int p = 0; p < M; p++) st[p][k] = counts[((size_t) clause * M + p) * wd + w];
+      st[M][k] = actions[(size_t) clause * wd + w];
+    } else {
+#pragma unroll
+      for (int p = 0; p <= M; p++) st[p][k] = 0;
+    }
+  }
+
+  auto recount = [&]() -> uint32_t {
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < WPL; k++) c += __popc(st[M][k] & word_mask(wbase + k, wd, P.last_eval));
+    return group_sum<G>(c);
+  };
+  uint32_t lit = recount();
+  lds_barrier();
+
+  const uint32_t T = P.target;
+  const uint32_t tol = P.tolerance;
+  // votes = max(0, cap - failed): a clause without literals fails nothing and
+  // votes tol, else min(literals, tol) - failed (capi.c:110-119); cap only
+  // changes with the literal count
+  auto cap_of = [&](uint32_t l) -> int32_t { return (int32_t)(l == 0 ? tol : (l < tol ? l : tol)); };
+  int32_t cap = cap_of(lit);
+  const int32_t vneg = -(int32_t)(j & 1);      // odd clauses vote against
+  const uint32_t T2 = 2 * T;
+  const bool narrow = T < 32768u;               // u16 * 2T fits in 32 bits
+  // lanes past the row's end read its last tile instead (masked by every use)
+  const uint32_t wload = min(wbase, P.row_stride - WPL);
+  uint32_t buf = 0;
+  uint32_t n_steps = 0, n_t1 = 0, n_grow = 0, n_t2 = 0, n_t1z = 0, n_draws = 0;
+
+  for (uint32_t e = 0; e < L.epochs; e++) {
+    const uint32_t epoch = epoch_first + e;
+    const uint32_t er = epoch - P.perm_epoch0;
+    const uint32_t *perm = P.perm + (size_t) er * L.n;
+    const uint32_t ek0 = P.ekeys[4 * er + 0], ek1 = P.ekeys[4 * er + 1];
+    const uint32_t ek2 = P.ekeys[4 * er + 2], ek3 = P.ekeys[4 * er + 3];
+
+    // Samples come in blocks of 64, one per lane: lane l holds the sample id,
+    // its target flag and its step-draw words.  `want` marks the lanes this
+    // chunk trains on.  Without a queue the chunk scans the epoch order itself
+    // (negative sampling applied here); with one, a block is the next 64 queue
+    // entries, loaded one block ahead.
+    uint32_t b0 = 0, bs = 0;
+    uint64_t want = 0, tmask = 0;
+    tm_u4 bd = { 0, 0, 0, 0 };
+    uint32_t bsel = 0;                 // full-wave group: this clause's draw word of bd
+    uint64_t qb = 0, qe = 0;           // queue entries not yet loaded
+    uint32_t qi_n = 0, qd_n = 0;       // prefetched block
+    tm_u4 qd4_n = { 0, 0, 0, 0 };
+    auto qload = [&]() {              // called with qb < qe
+      // lanes past the chunk's last entry re-read it: the load stays
+      // unconditional, so it can land in place without a wait
+      const uint64_t idx = min(qb + lane, qe - 1);
+      qi_n = L.qid[idx];
+      if constexpr (G == 64) {
+        qd_n = L.qdraw[(uint64_t)(j >> 1) * L.qcap + idx];
+      } else {
+        qd4_n.x = L.qdraw[idx];
+        qd4_n.y = L.qdraw[L.qcap + idx];
+        qd4_n.z = L.qdraw[2 * L.qcap + idx];
+        qd4_n.w = L.qdraw[3 * L.qcap + idx];
+      }
+    };
+    if constexpr (QUEUE) {
+      qb = L.qoff[(size_t) ch * L.qseg];
+      qe = L.qoff[(size_t)(ch + 1) * L.qseg];
+      if (qb < qe) qload();
+    }
+    auto scan = [&]() -> bool {       // advance to the next block with work
+      if constexpr (QUEUE) {
+        if (qb >= qe) return false;
+        const uint64_t left = qe - qb;
+        want = left >= 64 ? ~0ull : ((1ull << left) - 1);
+        bs = qi_n;
+        asm volatile("" : "+v"(bs));
+        if constexpr (G == 64) {
+          bsel = qd_n;
+          asm volatile("" : "+v"(bsel));
+        } else {
+          bd = qd4_n;
+        }
+        qb += 64;
+        if (qb < qe) qload();
+        return true;
+      } else {
+        while (b0 < L.n) {
+          const uint32_t pos = b0 + lane;
+          const bool valid = pos < L.n;
+          const uint32_t s = valid ? perm[pos] : 0u;
+          bool tgt, w;
+          if (L.encoder) {
+            tgt = valid && ((L.codes[(size_t) s * L.class_chunks + (cls >> 3)] >> (cls & 7)) & 1);
+            w = valid;
+          } else {
+            const uint32_t sc = valid ? (uint32_t) L.labels[s] : 0xFFFFFFFFu;
+            tgt = valid && sc == cls;
+            w = valid && (tgt || (uint64_t) tm_skip_draw(ek0, ek1, s, chr) >= P.skip_threshold);
+          }
+          b0 += 64;
+          want = __ballot(w);
+          if (want) {
+            tmask = __ballot(tgt);
+            bs = s;
+            bd = tm_philox(s, chr, epoch, TM_TAG_STEP, P.seed_lo, P.seed_hi);
+            if constexpr (G == 64) bsel = (j >> 1) == 0 ? bd.x : (j >> 1) == 1 ? bd.y : (j >> 1) == 2 ? bd.z : bd.w;
+            return true;
+          }
+        }
+        return false;
+      }
+    };
+    // take the next wanted sample of the current block (want != 0)
+    uint32_t nq_s = 0, nq_d = 0;
+    bool nq_t = false;
+    auto take = [&]() {
+      const int q = __builtin_ctzll(want);
+      want &= want - 1;
+      if constexpr (QUEUE) {
+        const uint32_t v = __builtin_amdgcn_readlane(bs, q);
+        nq_s = v & 0x7FFFFFFFu;
+        nq_t = v >> 31;
+      } else {
+        nq_s = __builtin_amdgcn_readlane(bs, q);
+        nq_t = (tmask >> q) & 1;
+      }
+      uint32_t dsel;
+      if constexpr (G == 64) {
+        dsel = __builtin_amdgcn_readlane(bsel, q);
+      } else {
+        const uint32_t d0 = __builtin_amdgcn_readlane(bd.x, q), d1 = __builtin_amdgcn_readlane(bd.y, q);
+        const uint32_t d2 = __builtin_amdgcn_readlane(bd.z, q), d3 = __builtin_amdgcn_readlane(bd.w, q);
+        dsel = (j >> 1) == 0 ? d0 : (j >> 1) == 1 ? d1 : (j >> 1) == 2 ? d2 : d3;
+      }
+      nq_d = (dsel >> (16 * (j & 1))) & 0xFFFFu;
+    };
+    uint32_t xn[WPL];
+    auto fetch = [&](uint32_t sample) { load_words<WPL>(P.rows + (size_t) sample * P.row_stride, wload, xn); };
+
+    if (!scan()) continue;
+    take();
+    fetch(nq_s);
+    for (;;) {
+      const uint32_t sq = nq_s, u16 = nq_d;
+      const bool tq = nq_t;
+      // x is defined here, by an empty asm on the copy: without it the
+      // compiler renames x onto the prefetch registers across its loop
+      // rotation and then waits for the next row's load (vmcnt(0)) before
+      // this step's evaluation, exposing the prefetch latency every step
+      uint32_t x[WPL];
+#pragma unroll
+      for (int k = 0; k < WPL; k++) {
+        x[k] = xn[k];
+        asm volatile("" : "+v"(x[k]));
+      }
+      // find the next sample and start loading its row before this one's work
+      const bool more = want ? true : scan();
+      if (more) {
+        take();
+        fetch(nq_s);
+      }
+
+      uint32_t f = 0;
+#pragma unroll
+      for (int k = 0; k < WPL; k++) f += __popc(st[M][k] & ~x[k] & word_mask(wbase + k, wd, P.last_eval));
+      f = group_sum<G>(f);
+      int32_t votes = cap - (int32_t) f;
+      if (votes < 0) votes = 0;
+      if (g == 0) votes_lds[buf][j] = (votes ^ vneg) - vneg;
+      lds_barrier();
+      int32_t v = 0;
+#pragma unroll
+      for (int c = 0; c < 8; c++) v += votes_lds[buf][c];
+      v = __builtin_amdgcn_readfirstlane(v);
+      buf ^= 1;
+      if (v > (int32_t) T) v = (int32_t) T;
+      if (v < -(int32_t) T) v = -(int32_t) T;
+      // fire iff u16 * 2T < (T -+ v) * 2^16; for integers x < num * 2^16
+      // exactly when x >> 16 < num, which stays in 32 bits when T < 2^15
+      const uint32_t num = (uint32_t)(tq ? (int32_t) T - v : (int32_t) T + v);
+      const bool fire = narrow ? ((u16 * T2) >> 16) < num : (uint64_t) u16 * T2 < ((uint64_t) num << 16);
+      n_steps++;
+      if (fire) {
+        const bool type1 = tq ^ (j & 1);
+        const bool out = votes > 0;
+        // kind 1: Type I on a true clause (capi.c:179-183): +1 on true
+        //   literals when literals <= clause_maximum, -1 on false literals
+        //   that are not included (the sets are disjoint, so "included" read
+        //   before the +1 is the same as after it);
+        // kind 2: Type I on a false clause (capi.c:184-192): s_draws random
+        //   single-literal decrements;
+        // kind 3: Type II on a true clause (capi.c:194-197): +1 on false
+        //   literals that are not included.
+        const int kind = type1 ? (out ? 1 : 2) : (out ? 3 : 0);
+        if (kind != 0) {
+          const bool grow = kind == 1 && lit <= P.maximum;
+          uint32_t kev = 0, nd = 0, base = 0, pend = 0;
+          if (kind == 1) { n_t1++; n_grow += grow; }
+          else if (kind == 3) n_t2++;
+          else {
+            // The draws are spread over the group's lanes (draw r on lane r % G).
+            // Saturating decrements commute, so each round applies one
+            // decrement to every distinct literal hit by the draws still
+            // pending; a draw whose literal was already hit this round waits
+            // for the next round.
+            kev = tm_literal_key(ek2, ek3, sq, clause_r);
+            nd = (P.s_draws + G - 1) / G;
+            n_t1z++;
+            n_draws += P.s_draws;
+          }
+          bool again, load = true;   // load: the group starts the batch of draws at `base`
+          uint32_t phase = 0;
+          do {
+            uint32_t mk[WPL], sgn;
+            if (kind == 2) {
+              if (load) {
+                // first pass over the batch: every lane's draws in order
+                // (a trip count uniform over the group), the hash input
+                // stepping by G*phi; a draw whose literal is already hit
+                // this round stays pending
+                const uint32_t cnt = min(32u, nd - base);
+                uint32_t r = base * G + g, hin = kev + r * TM_GOLDEN;
+                const uint32_t hstep = G * TM_GOLDEN;
+#pragma unroll 4
+                for (uint32_t i = 0; i < cnt; i++) {
+                  const bool valid = r < P.s_draws;
+                  const uint32_t lbit = tm_mulhi(tm_lowbias32(hin), P.input_bits);
+                  const uint32_t hb = valid ? 1u << (lbit & 31) : 0u;
+                  const uint32_t old = atomicOr(&hits[j][lbit >> 5], hb);
+                  if (old & hb) pend |= 1u << i;
+                  r += G;
+                  hin += hstep;
+                }
+                load = false;
+              } else {
+                uint32_t pp = pend;
+                while (pp) {
+                  const uint32_t i = __builtin_ctz(pp);
+                  pp &= pp - 1;
+                  const uint32_t lbit = tm_literal_draw(kev, (base + i) * G + g, P.input_bits);
+                  const uint32_t hb = 1u << (lbit & 31);
+                  const uint32_t old = atomicOr(&hits[j][lbit >> 5], hb);
+                  if (!(old & hb)) pend &= ~(1u << i);
+                }
+              }
+              __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
+#pragma unroll
+              for (int k = 0; k < WPL; k++)     // read and clear in one exchange
+                mk[k] = atomicExch(&hits[j][wbase + k], 0u) & any_one<M, WPL, M>(st, k);
+              __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
+              sgn = 0u;
+            } else if (kind == 1 && grow && phase == 0) {
+              // +1 on the true literals first, as tk_automata_inc precedes
+              // tk_automata_dec_not_excluded (capi.c:181-182)
+#pragma unroll
+              for (int k = 0; k < WPL; k++)
+                mk[k] = x[k] & ~all_ones<M, WPL>(st, k) & word_mask(wbase + k, wd, P.last_live);
+              sgn = ~0u;
+            } else if (kind == 3) {
+              // +1 on false literals that are not included; none is at the
+              // bound, since a counter with every plane one is included
+#pragma unroll
+              for (int k = 0; k < WPL; k++)
+                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & word_mask(wbase + k, wd, P.last_live);
+              sgn = ~0u;
+            } else {
+              // -1 on false literals that are not included and not at zero
+              // (with include_bits 1 their action plane is zero already)
+#pragma unroll
+              for (int k = 0; k < WPL; k++)
+                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & word_mask(wbase + k, wd, P.last_live) &
+                        any_one<M, WPL, IB1 ? M - 1 : M>(st, k);
+              sgn = 0u;
+            }
+            // one update site for every kind keeps the state in place (branch
+            // arms that each rewrite it make the compiler copy all of it)
+#pragma unroll
+            for (int k = 0; k < WPL; k++) ripple<M, WPL>(st, k, mk[k], sgn);
+            again = false;
+            if (kind == 2) {
+              // every lane of the group leaves a batch together, so no lane
+              // reloads draws that were already applied
+              const bool pending = group_any<G>(pend != 0);
+              if (!pending) {
+                base += 32;
+                load = true;
+              }
+              again = pending || base < nd;
+            } else if (kind == 1 && grow && phase == 0) {
+              again = true;
+            }
+            phase++;
+          } while (again);
+          // with include_bits 1, a Type I step without growth only lowers
+          // counters whose action bit is zero: the literal count stands
+          if (!(IB1 && kind == 1 && !grow)) {
+            lit = recount();
+            cap = cap_of(lit);
+          }
+        }
+      }
+      if (!more) break;
+    }
+  }
+
+  if (P.stats && g == 0) {
+    if (j == 0) atomicAdd(&P.stats[0], (unsigned long long) n_steps);
+    atomicAdd(&P.stats[1], (unsigned long long) n_t1);
+    atomicAdd(&P.stats[2], (unsigned long long) n_grow);
+    atomicAdd(&P.stats[3], (unsigned long long) n_t2);
+    atomicAdd(&P.stats[4], (unsigned long long) n_t1z);
+    atomicAdd(&P.stats[5], (unsigned long long) n_draws);
+  }
+
+#pragma unroll
+  for (int k = 0; k < WPL; k++) {
+    const uint32_t w = wbase + k;
+    if (w < wd) {
+#pragma unroll
+      for (int p = 0; p < M; p++) counts[((size_t) clause * M + p) * wd + w] = st[p][k];
+      actions[(size_t) clause * wd + w] = st[M][k];
+    }
+  }
+}
+
+} // namespace

@@ -144,9 +144,10 @@ def included(p, v):
 
 @pytest.mark.parametrize("features", [3, 5, 8, 63, 64, 100])
 @pytest.mark.parametrize("include_bits", [1, 3])
-def test_counter_ops_match_model(features, include_bits):
-    rng = np.random.default_rng(features * 10 + include_bits)
-    p = P(features=features, include_bits=include_bits)
+@pytest.mark.parametrize("state", [8, 2, 5, 16])
+def test_counter_ops_match_model(features, include_bits, state):
+    rng = np.random.default_rng(features * 10 + include_bits + 1000 * state)
+    p = P(features=features, include_bits=include_bits, state=state)
     L = O.lib()
     B = p.input_chunks
     m = p.state_bits - 1
@@ -178,11 +179,12 @@ def test_counter_ops_match_model(features, include_bits):
             a[bit] = True
             want = model_step(p, v[0], a, -1)
         got = planes_to_values(p, counts[: m * B], actions[:B])[0]
-        assert np.array_equal(got, want), (features, include_bits, op)
+        assert np.array_equal(got, want), (features, include_bits, state, op)
 
 
-def test_setup_value():
-    p = P(features=5)
+@pytest.mark.parametrize("state", [8, 2, 13])
+def test_setup_value(state):
+    p = P(features=5, state=state)
     m = O.Machine(p)
     m.actions[:] = 0xAB
     m.setup()
@@ -313,13 +315,14 @@ def py_train(p, rows, labels, codes, epochs):
     return v
 
 
-@pytest.mark.parametrize("features,kind,include_bits", [
-    (5, "classifier", 1), (8, "classifier", 2), (63, "classifier", 1), (12, "encoder", 1)])
-def test_replay_matches_python_trainer(features, kind, include_bits):
+@pytest.mark.parametrize("features,kind,include_bits,state", [
+    (5, "classifier", 1, 8), (8, "classifier", 2, 8), (63, "classifier", 1, 8), (12, "encoder", 1, 8),
+    (9, "classifier", 1, 2), (9, "classifier", 2, 3), (20, "encoder", 3, 12), (64, "classifier", 1, 16)])
+def test_replay_matches_python_trainer(features, kind, include_bits, state):
     rng = np.random.default_rng(features)
     classes = 3
     p = P(features=features, classes=classes, clauses=8, clause_tolerance=3, clause_maximum=4,
-          specificity=1.5, target=4, include_bits=include_bits, seed=11, type_=kind)
+          specificity=1.5, target=4, include_bits=include_bits, seed=11, type_=kind, state=state)
     n = 24
     xb = rng.random((n, features)) < 0.4
     from santoku.tsetlin.data import bits_to_cvec
