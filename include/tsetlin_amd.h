@@ -52,7 +52,9 @@ typedef struct {
   uint32_t clauses;          /* required, > 0; rounded up to a multiple of 8 */
   uint32_t clause_tolerance; /* required, > 0 */
   uint32_t clause_maximum;   /* required, > 0 */
-  uint32_t state;            /* "state" (state bits), default 8, >= 2 */
+  uint32_t state;            /* "state" (state bits), default 8, >= 2; at most 16
+                                here (TM_E_UNSUPPORTED above); rows up to 4096 bytes
+                                train at <= 8 state bits, up to 2048 bytes above */
   uint32_t include_bits;     /* default 1 (0 means 1) */
   double target;             /* default -1: sqrt(clauses/2)*tolerance */
   double negative;           /* default -1: 1/classes */
