@@ -248,6 +248,40 @@ int tm_last_queue_timing(const tm_machine *m, double *queue_ms);
 int tm_last_train_stats(const tm_machine *m, uint64_t out[TM_NSTATS]);
 
 /* ---------------------------------------------------------------------------
+ * Device-resident data preparation and dev metrics (device pointers in and
+ * out; each call runs on the legacy default stream and returns when done).
+ */
+
+/* ivec:bits_to_cvec(n, features, true) of santoku-matrix (layout
+ * doc/sth.md:85-91; the reference's callers, classify/mnist.lua:60): set_bits
+ * holds sample * features + feature for each present feature, in any order,
+ * duplicates allowed.  rows_dev receives n flip-interleaved rows of
+ * ceil(2 features / 8) bytes (x bits, then not-x bits, LSB first, padding
+ * zero).  Indices outside [0, n * features) are skipped and counted in
+ * *n_invalid (may be NULL). */
+int tm_bits_to_cvec_dev(const int64_t *set_bits_dev, uint64_t n_set, uint32_t n, uint32_t features, uint8_t *rows_dev,
+                        uint64_t *n_invalid);
+
+/* The same rows from bit-packed ones: raw row s is ceil(features / 8) bytes,
+ * bit f (LSB first) = feature f.  No reference counterpart (a dense form of
+ * the packer above). */
+int tm_flip_interleave_dev(const uint8_t *raw_dev, uint32_t n, uint32_t features, uint8_t *rows_dev);
+
+/* evaluator.class_accuracy -> tm_class_accuracy, evaluator/capi.c:119-203:
+ * per_class (3 * classes doubles: precision, recall, f1 per class; may be
+ * NULL) and macro (precision, recall, f1).  Predictions and labels are int64,
+ * converted to unsigned int as the reference does; samples with either
+ * outside [0, classes) are skipped. */
+int tm_class_accuracy_dev(const int64_t *predicted_dev, const int64_t *expected_dev, uint32_t n, uint32_t classes,
+                          double *per_class, double *macro);
+
+/* evaluator.encoding_accuracy -> tm_encoding_accuracy, evaluator/capi.c:497-574:
+ * bits (hidden doubles, per-bit error rate; may be NULL) and summary
+ * (mean_hamming, ber_min, ber_max, ber_std). */
+int tm_encoding_accuracy_dev(const uint8_t *predicted_dev, const uint8_t *expected_dev, uint32_t n, uint32_t hidden,
+                             double *bits, double *summary);
+
+/* ---------------------------------------------------------------------------
  * Batched search trials.  No reference counterpart: optimize.lua runs search
  * trials one after another on one reusable machine (optimize.lua:730-768).
  * Here N classifiers that share data, classes and features but differ in

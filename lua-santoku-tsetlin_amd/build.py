@@ -28,7 +28,8 @@ COMMON = [
 ]
 # (source, extra defines, object name); tm_train_states.hip is built once per
 # plane count other than 7 (state bits 2..16 except 8)
-SOURCES = [("tm_kernels.hip", [], "tm_kernels.o"), ("tm_api.cpp", [], "tm_api.o"), ("tm_mlp.hip", [], "tm_mlp.o")]
+SOURCES = [("tm_kernels.hip", [], "tm_kernels.o"), ("tm_api.cpp", [], "tm_api.o"), ("tm_mlp.hip", [], "tm_mlp.o"),
+           ("tm_data.hip", [], "tm_data.o")]
 SOURCES += [("tm_train_states.hip", ["-DTM_PLANES=%d" % m], "tm_train_planes_%d.o" % m)
             for m in range(1, 16) if m != 7]
 

@@ -92,6 +92,12 @@ uint32_t live_mask(uint32_t wd, uint32_t input_chunks, uint8_t tail_mask)
 
 } // namespace
 
+int tm_set_error(int code, const char *msg)
+{
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return code;
+}
+
 // One class of an individualized machine: a one-class machine over its own
 // feature set (capi.c:576-604), stored after the classes before it.
 struct tm_ind_class {

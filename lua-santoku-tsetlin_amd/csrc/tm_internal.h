@@ -90,6 +90,9 @@ struct tm_predict_launch {
   uint32_t pass_tiles;     // set by the launcher: 64-clause tiles per L2-resident pass
 };
 
+// Record `msg` as the calling thread's tm_last_error() text; returns code.
+int tm_set_error(int code, const char *msg);
+
 // Returns a hipError_t.  `ms` receives the time of the train kernels only.
 int tm_launch_train(const tm_train_launch &L, hipStream_t st);
 int tm_launch_setup(uint32_t *counts, uint32_t *actions, uint64_t nclauses, uint32_t m, uint32_t wd, hipStream_t st);

@@ -27,6 +27,7 @@ EXPORTS = [
     "tm_last_queue_timing",
     "tm_create_classifier_ind", "tm_create_encoder_ind", "tm_train_classifier_ind", "tm_train_encoder_ind",
     "tm_predict_classifier_ind", "tm_predict_encoder_ind", "tm_restrict", "tm_get_feature_sizes",
+    "tm_bits_to_cvec_dev", "tm_flip_interleave_dev", "tm_class_accuracy_dev", "tm_encoding_accuracy_dev",
     "tm_batch_create", "tm_batch_machine", "tm_batch_train_classifier", "tm_batch_train_classifier_dev",
     "tm_batch_accuracy_dev", "tm_batch_tuning", "tm_batch_timing", "tm_batch_destroy",
 ]
@@ -113,6 +114,12 @@ def load_library(path=LIB_PATH):
         "tm_predict_encoder_ind": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, i64p, u32, u8p]),
         "tm_restrict": (ctypes.c_int, [vp, i64p, u32]),
         "tm_get_feature_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64), u32]),
+        "tm_bits_to_cvec_dev": (ctypes.c_int, [vp, ctypes.c_uint64, u32, u32, vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "tm_flip_interleave_dev": (ctypes.c_int, [vp, u32, u32, vp]),
+        "tm_class_accuracy_dev": (ctypes.c_int, [vp, vp, u32, u32, ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_double)]),
+        "tm_encoding_accuracy_dev": (ctypes.c_int, [vp, vp, u32, u32, ctypes.POINTER(ctypes.c_double),
+                                                    ctypes.POINTER(ctypes.c_double)]),
         "tm_batch_create": (ctypes.c_int, [opt_p, u32, m_pp]),
         "tm_batch_machine": (ctypes.c_int, [vp, u32, m_pp]),
         "tm_batch_train_classifier": (ctypes.c_int, [vp, u32, u8p, i64p, u32, BATCH_EPOCH_FN, vp]),

@@ -115,3 +115,47 @@ def projection_codes_task(seed, n=500_000, visible=8192, hidden=256, density=0.1
         rows[s:e] = bits_to_cvec(xb)
         codes[s:e] = codes_from_bits(h)
     return dict(rows=rows, codes=codes, visible=visible, hidden=hidden)
+
+
+# ---------------------------------------------------------------------------
+# Device packers (the C ABI's tm_bits_to_cvec_dev / tm_flip_interleave_dev):
+# torch tensors on the GPU in, a uint8 tensor of rows out.
+# ---------------------------------------------------------------------------
+
+def _dev_lib():
+    from . import capi
+    return capi.load_library()
+
+
+def _dev_check(rc):
+    if rc != 0:
+        from . import TsetlinError, _c
+        raise TsetlinError("%s: %s" % (_c.ERRORS.get(rc, rc), _dev_lib().tm_last_error().decode()))
+
+
+def bits_to_cvec_dev(set_bits, n, features, out=None):
+    """ivec:bits_to_cvec(n, features, true) on the device: set_bits is an int64
+    tensor of sample * features + feature indices.  Returns (rows, invalid):
+    an (n, ceil(2F/8)) uint8 tensor and the count of indices outside
+    [0, n * features), which are skipped."""
+    import ctypes
+    import torch
+    set_bits = set_bits.contiguous()
+    if set_bits.dtype != torch.int64:
+        raise TypeError("set_bits must be int64")
+    if out is None:
+        out = torch.empty((n, (2 * features + 7) // 8), dtype=torch.uint8, device=set_bits.device)
+    bad = ctypes.c_uint64(0)
+    _dev_check(_dev_lib().tm_bits_to_cvec_dev(set_bits.data_ptr(), set_bits.numel(), n, features, out.data_ptr(),
+                                              ctypes.byref(bad)))
+    return out, bad.value
+
+
+def flip_interleave_dev(raw, n, features, out=None):
+    """Flip-interleaved rows from bit-packed rows (ceil(F/8) bytes, LSB first)."""
+    import torch
+    raw = raw.contiguous()
+    if out is None:
+        out = torch.empty((n, (2 * features + 7) // 8), dtype=torch.uint8, device=raw.device)
+    _dev_check(_dev_lib().tm_flip_interleave_dev(raw.data_ptr(), n, features, out.data_ptr()))
+    return out
