@@ -93,6 +93,8 @@ def _bind(L):
         "orc_s_draws": (ctypes.c_uint32, [P]),
         "orc_train_replay": (None, [P, _U8P, _U8P, ctypes.c_uint32, _U8P, _I64P, _U8P,
                                     ctypes.c_uint32, ctypes.c_uint32, _U64P]),
+        "orc_train_replay_chunks": (None, [P, _U8P, _U8P, ctypes.c_uint32, _U8P, _I64P, _U8P,
+                                    ctypes.c_uint32, ctypes.c_uint32, _U32P, ctypes.c_uint32, _U64P]),
         "orc_train_refshape": (None, [P, _U8P, _U8P, ctypes.c_uint32, _U8P, _I64P, _U8P,
                                       ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32]),
         "orc_max_threads": (ctypes.c_int, []),
@@ -159,6 +161,17 @@ class Machine:
         cod = None if codes is None else np.ascontiguousarray(codes, np.uint8)
         lib().orc_train_replay(ctypes.byref(self.p), ptr(self.counts), ptr(self.actions), n, ptr(rows),
                                ptr(lab, _I64P), ptr(cod), epoch_first, epochs, ptr(self.stats, _U64P))
+
+    def train_replay_chunks(self, rows, chunks, labels=None, codes=None, epochs=1, epoch_first=0):
+        """Replay of the listed chunks only (the others keep their state)."""
+        rows = np.ascontiguousarray(rows, np.uint8)
+        n = rows.shape[0]
+        lab = None if labels is None else np.ascontiguousarray(labels, np.int64)
+        cod = None if codes is None else np.ascontiguousarray(codes, np.uint8)
+        lst = np.ascontiguousarray(chunks, np.uint32)
+        lib().orc_train_replay_chunks(ctypes.byref(self.p), ptr(self.counts), ptr(self.actions), n, ptr(rows),
+                                      ptr(lab, _I64P), ptr(cod), epoch_first, epochs, ptr(lst, _U32P), lst.size,
+                                      ptr(self.stats, _U64P))
 
     def train_refshape(self, rows, labels=None, codes=None, epochs=1, threads=0, sample_limit=0, L=None):
         rows = np.ascontiguousarray(rows, np.uint8)

@@ -565,13 +565,19 @@ uint32_t orc_s_draws (const orc_params *p)
  * on its own 8 clauses), so they run in parallel here in any order.
  * epoch_first lets a caller replay epochs [epoch_first, epoch_first+epochs)
  * on an existing state (no setup when epoch_first > 0). */
-void orc_train_replay (const orc_params *p, uint8_t *counts, uint8_t *actions,
-                       uint32_t n, const uint8_t *rows, const int64_t *labels, const uint8_t *codes,
-                       uint32_t epoch_first, uint32_t epochs, uint64_t *stats)
+/* The replay of the chunks in `chunk_list` (all chunks when NULL); the other
+ * chunks' state is left as it is. */
+static void train_replay_list (const orc_params *p, uint8_t *counts, uint8_t *actions,
+                               uint32_t n, const uint8_t *rows, const int64_t *labels, const uint8_t *codes,
+                               uint32_t epoch_first, uint32_t epochs, const uint32_t *chunk_list,
+                               uint32_t n_list, uint64_t *stats)
 {
-  const uint32_t total = p->clause_chunks * p->classes;
+  const uint32_t total = chunk_list ? n_list : p->clause_chunks * p->classes;
   if (epoch_first == 0)
-    orc_setup(p, counts, actions, 0, (uint64_t) total * 8 - 1);
+    for (uint32_t i = 0; i < total; i++) {
+      uint32_t ch = chunk_list ? chunk_list[i] : i;
+      orc_setup(p, counts, actions, (uint64_t) ch * 8, (uint64_t) ch * 8 + 7);
+    }
   uint32_t *perm = malloc((size_t) n * sizeof(uint32_t) + 4);
   uint64_t thr = orc_threshold32(1.0 - p->negative);
   uint32_t s_draws = orc_s_draws(p);
@@ -584,7 +590,8 @@ void orc_train_replay (const orc_params *p, uint8_t *counts, uint8_t *actions,
     {
       uint64_t lst[ORC_NSTATS] = { 0 };
       #pragma omp for schedule(dynamic, 1)
-      for (uint32_t ch = 0; ch < total; ch++) {
+      for (uint32_t li = 0; li < total; li++) {
+        uint32_t ch = chunk_list ? chunk_list[li] : li;
         uint32_t cls = ch / p->clause_chunks;
         for (uint32_t i = 0; i < n; i++) {
           uint32_t s = perm[i];
@@ -606,6 +613,25 @@ void orc_train_replay (const orc_params *p, uint8_t *counts, uint8_t *actions,
   }
   if (stats) for (int k = 0; k < ORC_NSTATS; k++) stats[k] = st[k];
   free(perm);
+}
+
+void orc_train_replay (const orc_params *p, uint8_t *counts, uint8_t *actions,
+                       uint32_t n, const uint8_t *rows, const int64_t *labels, const uint8_t *codes,
+                       uint32_t epoch_first, uint32_t epochs, uint64_t *stats)
+{
+  train_replay_list(p, counts, actions, n, rows, labels, codes, epoch_first, epochs, NULL, 0, stats);
+}
+
+/* Replay of some chunks only.  Chunks are independent (capi.c:201-243: a
+ * chunk's feedback reads its own 8 votes), so each listed chunk ends exactly
+ * as in a replay of the whole machine, at a cost that does not grow with the
+ * number of classes and clauses. */
+void orc_train_replay_chunks (const orc_params *p, uint8_t *counts, uint8_t *actions,
+                              uint32_t n, const uint8_t *rows, const int64_t *labels, const uint8_t *codes,
+                              uint32_t epoch_first, uint32_t epochs, const uint32_t *chunk_list,
+                              uint32_t n_list, uint64_t *stats)
+{
+  train_replay_list(p, counts, actions, n, rows, labels, codes, epoch_first, epochs, chunk_list, n_list, stats);
 }
 
 /* ------------------------------------------------------------------------ */

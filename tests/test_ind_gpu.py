@@ -259,3 +259,21 @@ def test_ind_argument_errors():
     plain = tm.classifier(classes=2, features=8, clauses=8, clause_tolerance=3, clause_maximum=3, specificity=2)
     with pytest.raises(tm.TsetlinError, match="not individualized"):
         plain._predict_ind(prob, offs, n)
+
+
+def test_ind_zero_samples_and_iterations():
+    # setup still runs with no sample or no epoch (capi.c:1315-1327, :1413-1425)
+    widths = [6, 19]
+    rng = np.random.default_rng(61)
+    eng, orc = pair("classifier", widths, reusable=True)
+    acts = rng.integers(0, 256, eng.info()["action_chunks"], dtype=np.uint8)
+    eng.restore(acts)
+    prob, offs = ragged(widths, 10, rng)
+    eng.train(samples=0, problems=prob[:0], solutions=np.zeros(0, np.int64), dim_offsets=offs, iterations=3)
+    assert not eng.checkpoint().any()
+    eng.restore(acts)
+    eng.train(samples=10, problems=prob, solutions=np.zeros(10, np.int64), dim_offsets=offs, iterations=0)
+    assert not eng.checkpoint().any()
+    assert eng.predict(prob, offs, 0).shape == (0,)
+    enc, _ = pair("encoder", widths)
+    assert enc.predict(prob, offs, 0).shape == (0, 1)

@@ -372,3 +372,30 @@ def test_refshape_learns_an_easy_rule():
     m = O.Machine(p)
     m.train_refshape(d["train_rows"], labels=y[:2048], epochs=3, threads=4)
     assert (m.predict(d["test_rows"]) == y[2048:]).mean() > 0.97
+
+
+@pytest.mark.parametrize("kind", ["classifier", "encoder"])
+def test_chunk_replay_equals_whole_replay(kind):
+    # a chunk's feedback reads only its own 8 votes (capi.c:201-243), so a
+    # replay of some chunks gives those chunks of the whole machine's replay
+    rng = np.random.default_rng(77)
+    p = P(features=30, classes=5, clauses=24, clause_tolerance=4, clause_maximum=6, specificity=2.0,
+          target=5, seed=9, type_=kind, negative=0.3)
+    n = 120
+    from santoku.tsetlin.data import bits_to_cvec
+    rows = bits_to_cvec(rng.random((n, 30)) < 0.3)
+    labels = rng.integers(0, 5, n).astype(np.int64) if kind == "classifier" else None
+    codes = rng.integers(0, 32, (n, 1), dtype=np.uint8) if kind == "encoder" else None
+    whole = O.Machine(p)
+    whole.train_replay(rows, labels=labels, codes=codes, epochs=3)
+    part = O.Machine(p)
+    part.actions[:] = 0xEE
+    chunks = [14, 0, 7, 3]
+    part.train_replay_chunks(rows, chunks, labels=labels, codes=codes, epochs=3)
+    B = p.input_chunks
+    for ch in range(p.classes * p.clause_chunks):
+        sl = slice(ch * 8 * B, (ch + 1) * 8 * B)
+        if ch in chunks:
+            assert np.array_equal(part.actions[sl], whole.actions[sl]), ch
+        else:
+            assert (part.actions[sl] == 0xEE).all(), ch
