@@ -94,3 +94,20 @@ def test_c3_full_size_predict_batch_independent(c3_machine):
     pick = np.sort(rng.choice(n, 32, replace=False))
     host = rows[torch.from_numpy(pick).to(dev)].cpu().numpy()
     assert np.array_equal(full[torch.from_numpy(pick).to(dev)].cpu().numpy(), orc.predict(host))
+
+
+def test_c3_shape_train_longer_vs_chunk_replay(c3_machine):
+    # 2048 samples x 2 epochs at the full machine shape; the oracle replays a
+    # few chunks (chunks are independent), first and last hidden bits included
+    eng, orc = c3_machine
+    d = D.projection_codes_task(8, n=2048)
+    eng.train(samples=2048, sentences=d["rows"], codes=d["codes"], iterations=2)
+    chunks = [0, 127, 128 * 100 + 5, 128 * 200 + 64, 128 * 255 + 127]
+    orc.actions[:] = 0
+    orc.train_replay_chunks(d["rows"], chunks, codes=d["codes"], epochs=2)
+    assert orc.stats[1] > 0 and orc.stats[3] > 0
+    got = eng.checkpoint()
+    B = orc.p.input_chunks
+    for ch in chunks:
+        sl = slice(ch * 8 * B, (ch + 1) * 8 * B)
+        assert np.array_equal(got[sl], orc.actions[sl]), "chunk %d differs" % ch
