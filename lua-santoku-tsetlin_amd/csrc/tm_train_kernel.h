@@ -110,11 +110,15 @@ __device__ __forceinline__ void ripple(uint32_t (&st)[M + 1][WPL], int k, uint32
 #pragma unroll
   for (int p = 0; p <= M; p++) {
     const uint32_t s0 = st[p][k];
-    st[p][k] = s0 ^ m;
     // m & (s0 ^ ns) as one three-input op (truth table over m, s0, ns:
     // 0xF0 & (0xCC ^ 0xAA) = 0x60); left to itself the compiler spends an
-    // extra xor per plane here
-    if (p < M) m = __builtin_amdgcn_bitop3_b32(m, s0, ns, 0x60);
+    // extra xor per plane here.  The carry is taken before the plane is
+    // written, so the new plane can take the old one's register: written
+    // the other way round, the old plane outlives the new one and the
+    // compiler moves every plane back into place after each pass.
+    const uint32_t carry = p < M ? __builtin_amdgcn_bitop3_b32(m, s0, ns, 0x60) : 0u;
+    st[p][k] = s0 ^ m;
+    m = carry;
   }
 }
 
