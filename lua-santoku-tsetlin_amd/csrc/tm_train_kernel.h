@@ -138,17 +138,27 @@ __device__ __forceinline__ uint32_t included(const uint32_t (&st)[M + 1][WPL], i
 // Load WPL consecutive words of a literal row from word wbase.  Callers
 // clamp wbase into the row (the row stride is a multiple of WPL) so the load
 // is unconditional; words past the row's end are masked by every use.
+// Global-memory pointers.  Pointers that reach the kernel inside a struct
+// lose their address space, and loads through them become FLAT loads, which
+// count against lgkmcnt as well as vmcnt: the LDS-only barrier would then wait
+// for them.
+template <typename T> using gptr = const __attribute__((address_space(1))) T *;
+template <typename T> __device__ __forceinline__ gptr<T> global_ptr(const T *p) { return (gptr<T>) p; }
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
 template <int WPL>
-__device__ __forceinline__ void load_words(const uint32_t *row, uint32_t wbase, uint32_t (&x)[WPL])
+__device__ __forceinline__ void load_words(const uint32_t *row_generic, uint32_t wbase, uint32_t (&x)[WPL])
 {
+  const gptr<uint32_t> row = global_ptr(row_generic);
   if constexpr (WPL % 4 == 0) {
 #pragma unroll
     for (int k = 0; k < WPL; k += 4) {
-      const uint4 v = *reinterpret_cast<const uint4 *>(row + wbase + k);
+      const u32x4 v = *(gptr<u32x4>)(row + wbase + k);
       x[k] = v.x; x[k + 1] = v.y; x[k + 2] = v.z; x[k + 3] = v.w;
     }
   } else if constexpr (WPL == 2) {
-    const uint2 v = *reinterpret_cast<const uint2 *>(row + wbase);
+    const u32x2 v = *(gptr<u32x2>)(row + wbase);
     x[0] = v.x; x[1] = v.y;
   } else {
 #pragma unroll
@@ -256,7 +266,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
   for (uint32_t e = 0; e < L.epochs; e++) {
     const uint32_t epoch = epoch_first + e;
     const uint32_t er = epoch - P.perm_epoch0;
-    const uint32_t *perm = P.perm + (size_t) er * L.n;
+    const gptr<uint32_t> perm = global_ptr(P.perm + (size_t) er * L.n);
     const uint32_t ek0 = P.ekeys[4 * er + 0], ek1 = P.ekeys[4 * er + 1];
     const uint32_t ek2 = P.ekeys[4 * er + 2], ek3 = P.ekeys[4 * er + 3];
 
