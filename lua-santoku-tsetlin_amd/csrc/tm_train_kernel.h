@@ -257,7 +257,6 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
   int32_t cap = cap_of(lit);
   const int32_t vneg = -(int32_t)(j & 1);      // odd clauses vote against
   const uint32_t T2 = 2 * T;
-  const bool narrow = T < 32768u;               // u16 * 2T fits in 32 bits
   // lanes past the row's end read its last tile instead (masked by every use)
   const uint32_t wload = min(wbase, P.row_stride - WPL);
   uint32_t buf = 0;
@@ -400,7 +399,9 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
       f = group_sum<G>(f);
       int32_t votes = cap - (int32_t) f;
       if (votes < 0) votes = 0;
-      if (g == 0) votes_lds[buf][j] = (votes ^ vneg) - vneg;
+      // a full-wave group's vote is the same on every lane: all lanes store
+      // it (no exec-mask branch); smaller groups store from their lane 0
+      if (G == 64 || g == 0) votes_lds[buf][j] = (votes ^ vneg) - vneg;
       lds_barrier();
       int32_t v = 0;
 #pragma unroll
@@ -410,9 +411,11 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
       if (v > (int32_t) T) v = (int32_t) T;
       if (v < -(int32_t) T) v = -(int32_t) T;
       // fire iff u16 * 2T < (T -+ v) * 2^16; for integers x < num * 2^16
-      // exactly when x >> 16 < num, which stays in 32 bits when T < 2^15
+      // exactly when x >> 16 < num.  x >> 16 of the 48-bit product fits in
+      // 32 bits (T < 2^31), taken from its two 32-bit halves.
       const uint32_t num = (uint32_t)(tq ? (int32_t) T - v : (int32_t) T + v);
-      const bool fire = narrow ? ((u16 * T2) >> 16) < num : (uint64_t) u16 * T2 < ((uint64_t) num << 16);
+      const uint32_t prod_lo = u16 * T2, prod_hi = __umulhi(u16, T2);
+      const bool fire = ((prod_hi << 16) | (prod_lo >> 16)) < num;
       n_steps++;
       if (fire) {
         const bool type1 = tq ^ (j & 1);
