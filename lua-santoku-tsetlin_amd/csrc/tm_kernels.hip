@@ -120,16 +120,17 @@ __global__ __launch_bounds__(64 * PRED_WAVES) void predict_kernel(tm_predict_lau
 #pragma unroll
     for (int si = 0; si < PRED_SPW; si++) {
       const uint32_t sl = wave * PRED_SPW + si;
-      uint32_t t = acc[si];
+      // two accumulators keep two independent popcount chains in flight
+      uint32_t t0 = acc[si], t1 = 0;
 #pragma unroll
       for (int q = 0; q < PRED_WC / 4; q++) {
         const uint4 xv = xs[sl][q];
-        t += __popc(a[4 * q + 0] & xv.x);
-        t += __popc(a[4 * q + 1] & xv.y);
-        t += __popc(a[4 * q + 2] & xv.z);
-        t += __popc(a[4 * q + 3] & xv.w);
+        t0 = bcnt_acc(a[4 * q + 0] & xv.x, t0);
+        t1 = bcnt_acc(a[4 * q + 1] & xv.y, t1);
+        t0 = bcnt_acc(a[4 * q + 2] & xv.z, t0);
+        t1 = bcnt_acc(a[4 * q + 3] & xv.w, t1);
       }
-      acc[si] = t;
+      acc[si] = t0 + t1;
     }
   }
 

@@ -42,6 +42,16 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t v)
   return v;
 }
 
+// popcount(v) + acc as one v_bcnt_u32_b32.  Left to itself the compiler
+// turns a chain of popcount sums into popcounts of zero plus v_add3 trees, an
+// extra instruction for every two words.
+__device__ __forceinline__ uint32_t bcnt_acc(uint32_t v, uint32_t acc)
+{
+  uint32_t r;
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(v), "v"(acc));
+  return r;
+}
+
 // Workgroup barrier that orders LDS only: outstanding global loads (the
 // prefetched literal row) stay in flight across it.
 __device__ __forceinline__ void lds_barrier()
@@ -242,7 +252,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
   auto recount = [&]() -> uint32_t {
     uint32_t c = 0;
 #pragma unroll
-    for (int k = 0; k < WPL; k++) c += __popc(st[M][k] & word_mask(wbase + k, wd, P.last_eval));
+    for (int k = 0; k < WPL; k++) c = bcnt_acc(st[M][k] & word_mask(wbase + k, wd, P.last_eval), c);
     return group_sum<G>(c);
   };
   uint32_t lit = recount();
@@ -395,7 +405,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
 
       uint32_t f = 0;
 #pragma unroll
-      for (int k = 0; k < WPL; k++) f += __popc(st[M][k] & ~x[k] & word_mask(wbase + k, wd, P.last_eval));
+      for (int k = 0; k < WPL; k++) f = bcnt_acc(st[M][k] & ~x[k] & word_mask(wbase + k, wd, P.last_eval), f);
       f = group_sum<G>(f);
       int32_t votes = cap - (int32_t) f;
       if (votes < 0) votes = 0;
