@@ -90,6 +90,13 @@ uint32_t live_mask(uint32_t wd, uint32_t input_chunks, uint8_t tail_mask)
   return m;
 }
 
+// A trial whose rows are wd words with no bit of the last word masked: the
+// train kernel's variant for such rows keeps no masks.
+uint32_t full_rows(const tm_train_trial &Q, uint32_t wd)
+{
+  return Q.wd == wd && Q.last_live == 0xFFFFFFFFu && Q.last_eval == 0xFFFFFFFFu;
+}
+
 } // namespace
 
 int tm_set_error(int code, const char *msg)
@@ -812,6 +819,9 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
       L.trials = m->t_trials.as<tm_train_trial>();
       L.block_map = m->t_map.as<uint32_t>();
     }
+    L.all_full = 1;
+    if (T.empty()) L.all_full = full_rows(P, L.wd);
+    for (const tm_train_trial &Q : T) L.all_full &= full_rows(Q, L.wd);
     std::vector<uint32_t> ek(4 * group);
     bool stop = false;
     for (uint32_t e0 = 0; e0 < iterations && !stop; e0 += group) {
@@ -1528,7 +1538,11 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   L.last_live = m0->last_live_mask();
   L.last_eval = m0->last_eval_mask();
   L.all_ib1 = 1;
-  for (auto &P : T) L.all_ib1 &= P.include_bits <= 1;
+  L.all_full = 1;
+  for (auto &P : T) {
+    L.all_ib1 &= P.include_bits <= 1;
+    L.all_full &= full_rows(P, L.wd);
+  }
 
   auto upload_map = [&](uint32_t order) -> int {
     std::vector<uint32_t> map = batch_map(b, T, order);

@@ -299,11 +299,16 @@ template <int WPL, int G>
 int launch_train_shape(const tm_train_launch &L, hipStream_t st)
 {
   const dim3 grid(L.total_blocks), block(8 * G);
+  // rows that fill the full-wave tile exactly take the variant without tail
+  // masks (include_bits 1 only)
+  const bool full = G == 64 && L.all_ib1 && L.all_full && (uint64_t) G * WPL == L.wd;
   if (L.qid) {
-    if (L.all_ib1) hipLaunchKernelGGL((train_kernel<WPL, G, 7, true, true>), grid, block, 0, st, L);
+    if (full) hipLaunchKernelGGL((train_kernel<WPL, G, 7, true, true, G == 64>), grid, block, 0, st, L);
+    else if (L.all_ib1) hipLaunchKernelGGL((train_kernel<WPL, G, 7, true, true>), grid, block, 0, st, L);
     else hipLaunchKernelGGL((train_kernel<WPL, G, 7, false, true>), grid, block, 0, st, L);
   } else {
-    if (L.all_ib1) hipLaunchKernelGGL((train_kernel<WPL, G, 7, true, false>), grid, block, 0, st, L);
+    if (full) hipLaunchKernelGGL((train_kernel<WPL, G, 7, true, false, G == 64>), grid, block, 0, st, L);
+    else if (L.all_ib1) hipLaunchKernelGGL((train_kernel<WPL, G, 7, true, false>), grid, block, 0, st, L);
     else hipLaunchKernelGGL((train_kernel<WPL, G, 7, false, false>), grid, block, 0, st, L);
   }
   return (int) hipGetLastError();

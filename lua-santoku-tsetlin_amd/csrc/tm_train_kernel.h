@@ -194,7 +194,7 @@ __device__ __forceinline__ T to_sgpr(const T &v)
 // ---------------------------------------------------------------------------
 // training
 
-template <int WPL, int G, int M, bool IB1, bool QUEUE>
+template <int WPL, int G, int M, bool IB1, bool QUEUE, bool FULL = false>
 __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel(tm_train_launch L)
 {
   constexpr int TW = G * WPL;                  // words per clause tile
@@ -209,9 +209,9 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
   // depends on its class).  Rows are not shared through L2 anyway: every chunk
   // walks the epoch order at its own pace.
   uint32_t trial = 0, ch = blockIdx.x;
-  if (L.block_map) {
-    trial = L.block_map[2 * blockIdx.x];
-    ch = L.block_map[2 * blockIdx.x + 1];
+  if (L.block_map) {   // workgroup-uniform: held in scalar registers
+    trial = __builtin_amdgcn_readfirstlane(L.block_map[2 * blockIdx.x]);
+    ch = __builtin_amdgcn_readfirstlane(L.block_map[2 * blockIdx.x + 1]);
   }
   const tm_train_trial P = to_sgpr(L.trials ? L.trials[trial] : L.one);
   if (!P.active) return;
@@ -223,6 +223,12 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
   const uint32_t clause_r = chr * 8 + j;
   const uint32_t wbase = g * WPL;
   const uint32_t wd = P.wd;
+  // Mask of word k of this lane.  FULL: every trial's row fills the tile
+  // exactly (G * WPL words) and both last-word masks are all ones, so no word
+  // is masked and the masks take no registers.
+  auto wmask = [&](int k, uint32_t last) -> uint32_t {
+    return FULL ? 0xFFFFFFFFu : word_mask(wbase + k, wd, last);
+  };
   uint32_t *counts = P.counts;
   uint32_t *actions = P.actions;
 
@@ -239,7 +245,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
 #pragma unroll
       for (int p = 0; p < M; p++) st[p][k] = 0xFFFFFFFFu;
       st[M][k] = 0;
-    } else if (w < wd) {
+    } else if (FULL || w < wd) {
 #pragma unroll
       for (int p = 0; p < M; p++) st[p][k] = counts[((size_t) clause * M + p) * wd + w];
       st[M][k] = actions[(size_t) clause * wd + w];
@@ -252,7 +258,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
   auto recount = [&]() -> uint32_t {
     uint32_t c = 0;
 #pragma unroll
-    for (int k = 0; k < WPL; k++) c = bcnt_acc(st[M][k] & word_mask(wbase + k, wd, P.last_eval), c);
+    for (int k = 0; k < WPL; k++) c = bcnt_acc(st[M][k] & wmask(k, P.last_eval), c);
     return group_sum<G>(c);
   };
   uint32_t lit = recount();
@@ -268,9 +274,12 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
   const int32_t vneg = -(int32_t)(j & 1);      // odd clauses vote against
   const uint32_t T2 = 2 * T;
   // lanes past the row's end read its last tile instead (masked by every use)
-  const uint32_t wload = min(wbase, P.row_stride - WPL);
+  const uint32_t wload = FULL ? wbase : min(wbase, P.row_stride - WPL);
   uint32_t buf = 0;
   uint32_t n_steps = 0, n_t1 = 0, n_grow = 0, n_t2 = 0, n_t1z = 0, n_draws = 0;
+  // A full-wave group's counters are wave-uniform; said explicitly, they stay
+  // in scalar registers (the compiler otherwise keeps some of them in VGPRs).
+  auto uniform = [](uint32_t v) -> uint32_t { return G == 64 ? __builtin_amdgcn_readfirstlane(v) : v; };
 
   for (uint32_t e = 0; e < L.epochs; e++) {
     const uint32_t epoch = epoch_first + e;
@@ -405,7 +414,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
 
       uint32_t f = 0;
 #pragma unroll
-      for (int k = 0; k < WPL; k++) f = bcnt_acc(st[M][k] & ~x[k] & word_mask(wbase + k, wd, P.last_eval), f);
+      for (int k = 0; k < WPL; k++) f = bcnt_acc(st[M][k] & ~x[k] & wmask(k, P.last_eval), f);
       f = group_sum<G>(f);
       int32_t votes = cap - (int32_t) f;
       if (votes < 0) votes = 0;
@@ -442,7 +451,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
         if (kind != 0) {
           const bool grow = kind == 1 && lit <= P.maximum;
           uint32_t kev = 0, nd = 0, base = 0, pend = 0;
-          if (kind == 1) { n_t1++; n_grow += grow; }
+          if (kind == 1) { n_t1++; n_grow = uniform(n_grow + grow); }
           else if (kind == 3) n_t2++;
           else {
             // The draws are spread over the group's lanes (draw r on lane r % G).
@@ -452,8 +461,8 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
             // for the next round.
             kev = tm_literal_key(ek2, ek3, sq, clause_r);
             nd = (P.s_draws + G - 1) / G;
-            n_t1z++;
-            n_draws += P.s_draws;
+            n_t1z = uniform(n_t1z + 1);
+            n_draws = uniform(n_draws + P.s_draws);
           }
           bool again, load = true;   // load: the group starts the batch of draws at `base`
           uint32_t phase = 0;
@@ -501,21 +510,21 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
               // tk_automata_dec_not_excluded (capi.c:181-182)
 #pragma unroll
               for (int k = 0; k < WPL; k++)
-                mk[k] = x[k] & ~all_ones<M, WPL>(st, k) & word_mask(wbase + k, wd, P.last_live);
+                mk[k] = x[k] & ~all_ones<M, WPL>(st, k) & wmask(k, P.last_live);
               sgn = ~0u;
             } else if (kind == 3) {
               // +1 on false literals that are not included; none is at the
               // bound, since a counter with every plane one is included
 #pragma unroll
               for (int k = 0; k < WPL; k++)
-                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & word_mask(wbase + k, wd, P.last_live);
+                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & wmask(k, P.last_live);
               sgn = ~0u;
             } else {
               // -1 on false literals that are not included and not at zero
               // (with include_bits 1 their action plane is zero already)
 #pragma unroll
               for (int k = 0; k < WPL; k++)
-                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & word_mask(wbase + k, wd, P.last_live) &
+                mk[k] = ~x[k] & ~included<M, WPL, IB1>(st, k, P.include_bits) & wmask(k, P.last_live) &
                         any_one<M, WPL, IB1 ? M - 1 : M>(st, k);
               sgn = 0u;
             }
@@ -562,7 +571,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
 #pragma unroll
   for (int k = 0; k < WPL; k++) {
     const uint32_t w = wbase + k;
-    if (w < wd) {
+    if (FULL || w < wd) {
 #pragma unroll
       for (int p = 0; p < M; p++) counts[((size_t) clause * M + p) * wd + w] = st[p][k];
       actions[(size_t) clause * wd + w] = st[M][k];
