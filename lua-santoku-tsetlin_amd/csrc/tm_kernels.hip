@@ -300,10 +300,13 @@ int launch_train_shape(const tm_train_launch &L, hipStream_t st)
 {
   const dim3 grid(L.total_blocks), block(8 * G);
   // rows that fill the full-wave tile exactly take the variant without tail
-  // masks (include_bits 1 only)
+  // masks (include_bits 1 only); up to 4 words per lane it runs at four
+  // workgroups per CU
   const bool full = G == 64 && L.all_ib1 && L.all_full && (uint64_t) G * WPL == L.wd;
+  constexpr bool R4 = G == 64 && WPL <= 4;   // step-queue mode only
   if (L.qid) {
-    if (full) hipLaunchKernelGGL((train_kernel<WPL, G, 7, true, true, G == 64>), grid, block, 0, st, L);
+    if (full && R4) hipLaunchKernelGGL((train_kernel_r4<R4 ? WPL : 1, true>), grid, block, 0, st, L);
+    else if (full) hipLaunchKernelGGL((train_kernel<WPL, G, 7, true, true, G == 64>), grid, block, 0, st, L);
     else if (L.all_ib1) hipLaunchKernelGGL((train_kernel<WPL, G, 7, true, true>), grid, block, 0, st, L);
     else hipLaunchKernelGGL((train_kernel<WPL, G, 7, false, true>), grid, block, 0, st, L);
   } else {

@@ -191,11 +191,42 @@ __device__ __forceinline__ T to_sgpr(const T &v)
   return out;
 }
 
+// Event counters of a clause group (Type I on a true clause, of which grew,
+// Type II, Type I on a false clause, random-literal draws).  Smaller groups
+// differ per lane and keep one register per counter.  A full-wave group's
+// counters are wave-uniform: in scalar registers (said explicitly; the
+// compiler otherwise keeps some in VGPRs), or with IN_LDS in LDS, where every
+// lane of the wave writes the same sum to the same slot and no register is
+// held.
+template <int G, bool IN_LDS>
+struct event_counters {
+  uint32_t c[5] = { 0, 0, 0, 0, 0 };
+  __device__ __forceinline__ void init(unsigned long long *) {}
+  __device__ __forceinline__ void add(int i, uint32_t d)
+  {
+    c[i] = G == 64 ? __builtin_amdgcn_readfirstlane(c[i] + d) : c[i] + d;
+  }
+  __device__ __forceinline__ unsigned long long get(int i) const { return c[i]; }
+};
+
+template <>
+struct event_counters<64, true> {
+  unsigned long long *slot;     // this clause's 5 counters in LDS, zeroed by init
+  __device__ __forceinline__ void init(unsigned long long *s)
+  {
+    slot = s;
+#pragma unroll
+    for (int i = 0; i < 5; i++) slot[i] = 0;
+  }
+  __device__ __forceinline__ void add(int i, uint32_t d) { slot[i] += d; }
+  __device__ __forceinline__ unsigned long long get(int i) const { return slot[i]; }
+};
+
 // ---------------------------------------------------------------------------
 // training
 
-template <int WPL, int G, int M, bool IB1, bool QUEUE, bool FULL = false>
-__global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel(tm_train_launch L)
+template <int WPL, int G, int M, bool IB1, bool QUEUE, bool FULL, bool CNT_LDS>
+__device__ __forceinline__ void train_body(const tm_train_launch &L)
 {
   constexpr int TW = G * WPL;                  // words per clause tile
   const uint32_t tid = threadIdx.x;
@@ -276,10 +307,10 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
   // lanes past the row's end read its last tile instead (masked by every use)
   const uint32_t wload = FULL ? wbase : min(wbase, P.row_stride - WPL);
   uint32_t buf = 0;
-  uint32_t n_steps = 0, n_t1 = 0, n_grow = 0, n_t2 = 0, n_t1z = 0, n_draws = 0;
-  // A full-wave group's counters are wave-uniform; said explicitly, they stay
-  // in scalar registers (the compiler otherwise keeps some of them in VGPRs).
-  auto uniform = [](uint32_t v) -> uint32_t { return G == 64 ? __builtin_amdgcn_readfirstlane(v) : v; };
+  uint32_t n_steps = 0;
+  __shared__ unsigned long long ev_lds[CNT_LDS ? 8 : 1][5];
+  event_counters<G, CNT_LDS> cnt;
+  cnt.init(ev_lds[CNT_LDS ? j : 0]);
 
   for (uint32_t e = 0; e < L.epochs; e++) {
     const uint32_t epoch = epoch_first + e;
@@ -451,8 +482,10 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
         if (kind != 0) {
           const bool grow = kind == 1 && lit <= P.maximum;
           uint32_t kev = 0, nd = 0, base = 0, pend = 0;
-          if (kind == 1) { n_t1++; n_grow = uniform(n_grow + grow); }
-          else if (kind == 3) n_t2++;
+          if (kind == 1) {
+            cnt.add(0, 1);
+            if (grow) cnt.add(1, 1);
+          } else if (kind == 3) cnt.add(2, 1);
           else {
             // The draws are spread over the group's lanes (draw r on lane r % G).
             // Saturating decrements commute, so each round applies one
@@ -461,8 +494,8 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
             // for the next round.
             kev = tm_literal_key(ek2, ek3, sq, clause_r);
             nd = (P.s_draws + G - 1) / G;
-            n_t1z = uniform(n_t1z + 1);
-            n_draws = uniform(n_draws + P.s_draws);
+            cnt.add(3, 1);
+            cnt.add(4, P.s_draws);
           }
           bool again, load = true;   // load: the group starts the batch of draws at `base`
           uint32_t phase = 0;
@@ -561,11 +594,8 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
 
   if (P.stats && g == 0) {
     if (j == 0) atomicAdd(&P.stats[0], (unsigned long long) n_steps);
-    atomicAdd(&P.stats[1], (unsigned long long) n_t1);
-    atomicAdd(&P.stats[2], (unsigned long long) n_grow);
-    atomicAdd(&P.stats[3], (unsigned long long) n_t2);
-    atomicAdd(&P.stats[4], (unsigned long long) n_t1z);
-    atomicAdd(&P.stats[5], (unsigned long long) n_draws);
+#pragma unroll
+    for (int i = 0; i < 5; i++) atomicAdd(&P.stats[1 + i], cnt.get(i));
   }
 
 #pragma unroll
@@ -577,6 +607,25 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
       actions[(size_t) clause * wd + w] = st[M][k];
     }
   }
+}
+
+template <int WPL, int G, int M, bool IB1, bool QUEUE, bool FULL = false>
+__global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel(tm_train_launch L)
+{
+  train_body<WPL, G, M, IB1, QUEUE, FULL, false>(L);
+}
+
+// The full-wave tile of up to 4 words per lane, include_bits 1, rows without
+// tail masks, step-queue mode, at four workgroups (32 waves) per CU.  That
+// takes at most 64 VGPRs and few enough SGPRs: a CU of gfx950 holds four
+// 512-thread workgroups of 64 VGPRs at 70 SGPRs each but only three at 86, so
+// the kernel is held to 76 SGPRs (the compiler keeps the rest in the lanes of
+// a VGPR) and its event counters live in LDS.  The 4-word tile then takes 63
+// VGPRs by itself; a 64-VGPR launch bound instead makes the compiler spill.
+template <int WPL, bool QUEUE>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_num_sgpr(76))) void train_kernel_r4(tm_train_launch L)
+{
+  train_body<WPL, 64, 7, true, QUEUE, true, true>(L);
 }
 
 } // namespace
