@@ -543,7 +543,7 @@ def main():
             "unit": "Tops/s (int32 lane-ops)",
             "frac": round(achieved / VALU_PEAK_TOPS, 4),
             "traffic": profiled_traffic(),
-            "kernel": "train_kernel<4,64,7,ib1,queue> (one launch per epoch)",
+            "kernel": "train_kernel_r4<4,queue> (one launch per epoch)",
             "launches": args.steps,
             "ops_per_launch": ops // args.steps,
             "eval_ops_per_launch": eval_ops // args.steps,
