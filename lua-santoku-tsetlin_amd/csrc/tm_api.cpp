@@ -138,7 +138,7 @@ struct tm_machine {
   dbuf t_trials, t_map;                                                       // individualized train table
   dbuf p_rows, p_words, p_sums, p_out;                                        // predict data
   dbuf t_stats;
-  dbuf q_counts, q_offsets, q_scratch, q_id, q_draw;                          // step queue
+  dbuf q_counts, q_offsets, q_scratch, q_id, q_draw, q_keys;                  // step queue
   uint64_t q_cap = 0;
   hipEvent_t evq0 = nullptr, evq1 = nullptr;
   uint64_t stats[TM_NSTATS] = { 0 };
@@ -612,7 +612,7 @@ int build_queue(tm_machine *m, tm_train_launch &L, const uint32_t ek[4], uint32_
   if (items > 0x7FFFFFFFull) return TM_OK;
   int rc;
   if ((rc = m->q_counts.ensure(items * 8)) || (rc = m->q_offsets.ensure(items * 8)) ||
-      (rc = m->q_scratch.ensure(tm_queue_scratch_bytes(items))))
+      (rc = m->q_scratch.ensure(tm_queue_scratch_bytes(items))) || (rc = m->q_keys.ensure((size_t) L.n * 8)))
     return rc;
   tm_queue_launch Q;
   memset(&Q, 0, sizeof Q);
@@ -630,6 +630,7 @@ int build_queue(tm_machine *m, tm_train_launch &L, const uint32_t ek[4], uint32_
   Q.skip_threshold = L.one.skip_threshold;
   Q.counts = m->q_counts.as<uint64_t>();
   Q.offsets = m->q_offsets.as<uint64_t>();
+  Q.pos_keys = m->q_keys.as<uint32_t>();
   HIP_TRY(hipEventRecord(m->evq0, m->stream));
   HIP_TRY(tm_launch_queue_count(Q, m->q_scratch.p, m->q_scratch.cap, m->stream));
   uint64_t total = 0;
@@ -872,6 +873,7 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
   // the queue is sized by the data: give its memory back between calls
   m->q_id.release();
   m->q_draw.release();
+  m->q_keys.release();
   m->q_cap = 0;
   // capi.c:1187-1189: a non-reusable machine releases its counters
   if (!m->reusable) {
@@ -1339,7 +1341,7 @@ void tm_destroy(tm_machine *m)
   dbuf *bufs[] = { &m->counts, &m->actions, &m->actT, &m->lit, &m->t_rows, &m->t_words, &m->t_labels,
                    &m->t_codes, &m->t_perm, &m->t_perm_scratch, &m->t_ekeys, &m->p_rows, &m->p_words,
                    &m->p_sums, &m->p_out, &m->t_stats, &m->q_counts, &m->q_offsets, &m->q_scratch,
-                   &m->q_id, &m->q_draw, &m->t_trials, &m->t_map };
+                   &m->q_id, &m->q_draw, &m->q_keys, &m->t_trials, &m->t_map };
   for (dbuf *b : bufs) b->release();
   if (m->ev0) hipEventDestroy(m->ev0);
   if (m->ev1) hipEventDestroy(m->ev1);

@@ -75,6 +75,9 @@ struct tm_queue_launch {
   uint64_t *offsets;              // [chunks * nseg + 1], exclusive sums; the last is the total
   uint32_t *qid, *qdraw;
   uint64_t qcap;
+  // per epoch position: the sample's half of its skip hash and its label as
+  // uint32 [2][n], written by the count launch and read by both passes
+  uint32_t *pos_keys;
 };
 
 // Work counters of a train launch: TM_NSTATS entries, as documented at

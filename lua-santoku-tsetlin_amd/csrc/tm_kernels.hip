@@ -233,14 +233,28 @@ constexpr uint32_t kQueueSeg = 1024;   // epoch positions per (chunk, segment) w
 
 // Whether chunk ch trains on epoch position pos: the sample is of the chunk's
 // class, or a negative that the skip draw keeps (the in-kernel scan's test).
+// Per epoch position: the inner hash of the skip draw (it depends on the
+// sample only) and the sample's label, so the passes over (chunk, position)
+// read both in order instead of gathering labels per chunk.
+__global__ __launch_bounds__(256) void queue_keys_kernel(tm_queue_launch Q)
+{
+  const uint32_t pos = blockIdx.x * 256 + threadIdx.x;
+  if (pos >= Q.n) return;
+  const uint32_t s = Q.perm[pos];
+  Q.pos_keys[pos] = tm_lowbias32(s ^ Q.ek0);
+  Q.pos_keys[Q.n + pos] = (uint32_t) Q.labels[s];
+}
+
+// tm_skip_draw(ek0, ek1, s, ch) from the sample's precomputed inner hash
 __device__ __forceinline__ bool queue_wanted(const tm_queue_launch &Q, uint32_t pos, uint32_t ch, uint32_t cls,
                                              uint32_t &s, bool &tgt)
 {
   const bool valid = pos < Q.n;
   s = valid ? Q.perm[pos] : 0u;
-  const uint32_t sc = valid ? (uint32_t) Q.labels[s] : 0xFFFFFFFFu;
+  const uint32_t key = valid ? Q.pos_keys[pos] : 0u;
+  const uint32_t sc = valid ? Q.pos_keys[Q.n + pos] : 0xFFFFFFFFu;
   tgt = valid && sc == cls;
-  return valid && (tgt || (uint64_t) tm_skip_draw(Q.ek0, Q.ek1, s, ch) >= Q.skip_threshold);
+  return valid && (tgt || (uint64_t) tm_lowbias32(key ^ (ch * TM_GOLDEN + Q.ek1)) >= Q.skip_threshold);
 }
 
 // grid (chunks, ceil(nseg / 4)), 4 waves per block, one (chunk, segment) each
@@ -478,8 +492,10 @@ size_t tm_queue_scratch_bytes(uint64_t items)
 int tm_launch_queue_count(const tm_queue_launch &Q, void *scratch, size_t scratch_bytes, hipStream_t st)
 {
   const uint64_t items = (uint64_t) Q.chunks * Q.nseg + 1;
-  if (Q.n == 0 || Q.nseg != tm_queue_segments(Q.n) || items > 0x7FFFFFFFull || (Q.nseg + 3) / 4 > 65535)
+  if (Q.n == 0 || !Q.pos_keys || Q.nseg != tm_queue_segments(Q.n) || items > 0x7FFFFFFFull ||
+      (Q.nseg + 3) / 4 > 65535)
     return (int) hipErrorInvalidValue;
+  hipLaunchKernelGGL(queue_keys_kernel, dim3((Q.n + 255) / 256), dim3(256), 0, st, Q);
   hipLaunchKernelGGL(queue_count_kernel, dim3(Q.chunks, (Q.nseg + 3) / 4), dim3(256), 0, st, Q);
   size_t tb = scratch_bytes;
   hipError_t err = hipcub::DeviceScan::ExclusiveSum(scratch, tb, Q.counts, Q.offsets, (int) items, st);
@@ -489,7 +505,7 @@ int tm_launch_queue_count(const tm_queue_launch &Q, void *scratch, size_t scratc
 
 int tm_launch_queue_write(const tm_queue_launch &Q, hipStream_t st)
 {
-  if (!Q.qid || !Q.qdraw || Q.nseg != tm_queue_segments(Q.n) || (Q.nseg + 3) / 4 > 65535)
+  if (!Q.qid || !Q.qdraw || !Q.pos_keys || Q.nseg != tm_queue_segments(Q.n) || (Q.nseg + 3) / 4 > 65535)
     return (int) hipErrorInvalidValue;
   hipLaunchKernelGGL(queue_write_kernel, dim3(Q.chunks, (Q.nseg + 3) / 4), dim3(256), 0, st, Q);
   return (int) hipGetLastError();
