@@ -99,6 +99,16 @@ __device__ __forceinline__ uint32_t all_ones(const uint32_t (&st)[M + 1][WPL], i
   return v;
 }
 
+// some plane FROM..M one: the counter is at least 2^FROM
+template <int M, int WPL, int FROM>
+__device__ __forceinline__ uint32_t any_one_from(const uint32_t (&st)[M + 1][WPL], int k)
+{
+  uint32_t v = 0;
+#pragma unroll
+  for (int p = FROM; p <= M; p++) v |= st[p][k];
+  return v;
+}
+
 template <int M, int WPL, int TOP>
 __device__ __forceinline__ uint32_t any_one(const uint32_t (&st)[M + 1][WPL], int k)
 {
@@ -129,6 +139,30 @@ __device__ __forceinline__ void ripple(uint32_t (&st)[M + 1][WPL], int k, uint32
     const uint32_t carry = p < M ? __builtin_amdgcn_bitop3_b32(m, s0, ns, 0x60) : 0u;
     st[p][k] = s0 ^ m;
     m = carry;
+  }
+}
+
+// The same with a two-bit amount: +-(m0 + 2 m1) on each literal, where the
+// caller has clamped the amount so that no counter leaves its range.  Plane 1
+// takes the amount's second bit in a full adder (borrow: majority of the
+// complemented plane, m1 and the incoming borrow).
+template <int M, int WPL>
+__device__ __forceinline__ void ripple2(uint32_t (&st)[M + 1][WPL], int k, uint32_t m0, uint32_t m1, uint32_t sgn)
+{
+  const uint32_t ns = ~sgn;
+  const uint32_t s0 = st[0][k];
+  uint32_t m = __builtin_amdgcn_bitop3_b32(m0, s0, ns, 0x60);
+  st[0][k] = s0 ^ m0;
+  const uint32_t s1 = st[1][k];
+  const uint32_t carry = M > 1 ? __builtin_amdgcn_bitop3_b32(s1 ^ ns, m1, m, 0xE8) : 0u;   // majority
+  st[1][k] = __builtin_amdgcn_bitop3_b32(s1, m1, m, 0x96);                                 // xor3
+  m = carry;
+#pragma unroll
+  for (int p = 2; p <= M; p++) {
+    const uint32_t sp = st[p][k];
+    const uint32_t c = p < M ? __builtin_amdgcn_bitop3_b32(m, sp, ns, 0x60) : 0u;
+    st[p][k] = sp ^ m;
+    m = c;
   }
 }
 
@@ -225,7 +259,7 @@ struct event_counters<64, true> {
 // ---------------------------------------------------------------------------
 // training
 
-template <int WPL, int G, int M, bool IB1, bool QUEUE, bool FULL, bool CNT_LDS>
+template <int WPL, int G, int M, bool IB1, bool QUEUE, bool FULL, bool CNT_LDS, bool MULTI>
 __device__ __forceinline__ void train_body(const tm_train_launch &L)
 {
   constexpr int TW = G * WPL;                  // words per clause tile
@@ -264,9 +298,12 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
   uint32_t *actions = P.actions;
 
   __shared__ int32_t votes_lds[2][8];
-  __shared__ uint32_t hits[8][TW];
+  // bits of the literals hit by random-literal draws, per clause; with
+  // MULTI, on three levels: hit once (0), a second time (1), a third (2)
+  constexpr int HL = MULTI ? 3 : 1;
+  __shared__ uint32_t hits[HL][8][TW];
 
-  for (uint32_t i = tid; i < 8 * TW; i += 8 * G) (&hits[0][0])[i] = 0;
+  for (uint32_t i = tid; i < HL * 8 * TW; i += 8 * G) (&hits[0][0][0])[i] = 0;
 
   uint32_t st[M + 1][WPL];
 #pragma unroll
@@ -500,8 +537,13 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
           bool again, load = true;   // load: the group starts the batch of draws at `base`
           uint32_t phase = 0;
           do {
-            uint32_t mk[WPL], sgn;
+            uint32_t mk[WPL], m1[MULTI ? WPL : 1], sgn;
+            if constexpr (MULTI) {
+#pragma unroll
+              for (int k = 0; k < WPL; k++) m1[k] = 0u;
+            }
             if (kind == 2) {
+              uint32_t levels = 1;     // hit levels this pass applies
               if (load) {
                 // first pass over the batch: every lane's draws in order
                 // (a trip count uniform over the group), the hash input
@@ -515,12 +557,28 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
                   const bool valid = r < P.s_draws;
                   const uint32_t lbit = tm_mulhi(tm_lowbias32(hin), P.input_bits);
                   const uint32_t hb = valid ? 1u << (lbit & 31) : 0u;
-                  const uint32_t old = atomicOr(&hits[j][lbit >> 5], hb);
+                  const uint32_t old = atomicOr(&hits[0][j][lbit >> 5], hb);
                   if (old & hb) pend |= 1u << i;
                   r += G;
                   hin += hstep;
                 }
                 load = false;
+                if constexpr (MULTI) {
+                  // a draw whose literal was already hit is drawn again onto
+                  // the next level; one that finds all three levels hit
+                  // stays pending for a round of its own
+                  while (levels < HL && group_any<G>(pend != 0)) {
+                    uint32_t pp = pend;
+                    while (pp) {
+                      const uint32_t i = __builtin_ctz(pp);
+                      pp &= pp - 1;
+                      const uint32_t lbit = tm_literal_draw(kev, (base + i) * G + g, P.input_bits);
+                      const uint32_t hb = 1u << (lbit & 31);
+                      if (!(atomicOr(&hits[levels][j][lbit >> 5], hb) & hb)) pend &= ~(1u << i);
+                    }
+                    levels++;
+                  }
+                }
               } else {
                 uint32_t pp = pend;
                 while (pp) {
@@ -528,14 +586,32 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
                   pp &= pp - 1;
                   const uint32_t lbit = tm_literal_draw(kev, (base + i) * G + g, P.input_bits);
                   const uint32_t hb = 1u << (lbit & 31);
-                  const uint32_t old = atomicOr(&hits[j][lbit >> 5], hb);
+                  const uint32_t old = atomicOr(&hits[0][j][lbit >> 5], hb);
                   if (!(old & hb)) pend &= ~(1u << i);
                 }
               }
               __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
 #pragma unroll
-              for (int k = 0; k < WPL; k++)     // read and clear in one exchange
-                mk[k] = atomicExch(&hits[j][wbase + k], 0u) & any_one<M, WPL, M>(st, k);
+              for (int k = 0; k < WPL; k++) {   // read and clear in one exchange
+                if constexpr (MULTI) {
+                  // levels nest, so a literal was hit h0 + h1 + h2 times: a
+                  // two-bit count (k1 k0), clamped to the counter (a counter
+                  // below 4 is the two bits c1 c0) so that the subtraction
+                  // saturates at zero
+                  const uint32_t h0 = atomicExch(&hits[0][j][wbase + k], 0u);
+                  const uint32_t h1 = levels > 1 ? atomicExch(&hits[1][j][wbase + k], 0u) : 0u;
+                  const uint32_t h2 = levels > 2 ? atomicExch(&hits[2][j][wbase + k], 0u) : 0u;
+                  const uint32_t k0 = h0 ^ h1 ^ h2, k1 = h1;
+                  const uint32_t c0 = st[0][k], c1 = st[1][k];
+                  const uint32_t hi = M > 1 ? any_one_from<M, WPL, 2>(st, k) : 0u;
+                  // count above counter: k1 > c1, or k1 == c1 and k0 > c0
+                  const uint32_t gt = ~hi & ((k1 & ~c1) | (~(k1 ^ c1) & k0 & ~c0));
+                  mk[k] = (gt & c0) | (~gt & k0);
+                  m1[k] = (gt & c1) | (~gt & k1);
+                } else {
+                  mk[k] = atomicExch(&hits[0][j][wbase + k], 0u) & any_one<M, WPL, M>(st, k);
+                }
+              }
               __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
               sgn = 0u;
             } else if (kind == 1 && grow && phase == 0) {
@@ -564,7 +640,10 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
             // one update site for every kind keeps the state in place (branch
             // arms that each rewrite it make the compiler copy all of it)
 #pragma unroll
-            for (int k = 0; k < WPL; k++) ripple<M, WPL>(st, k, mk[k], sgn);
+            for (int k = 0; k < WPL; k++) {
+              if constexpr (MULTI) ripple2<M, WPL>(st, k, mk[k], m1[k], sgn);
+              else ripple<M, WPL>(st, k, mk[k], sgn);
+            }
             again = false;
             if (kind == 2) {
               // every lane of the group leaves a batch together, so no lane
@@ -612,7 +691,10 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
 template <int WPL, int G, int M, bool IB1, bool QUEUE, bool FULL = false>
 __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel(tm_train_launch L)
 {
-  train_body<WPL, G, M, IB1, QUEUE, FULL, false>(L);
+  // the multi-level draws' second amount register per word fits where the
+  // state takes at most 32 VGPRs (64 without tail masks); larger tiles spill
+  constexpr bool MULTI = (M + 1) * WPL <= (FULL ? 64 : 32);
+  train_body<WPL, G, M, IB1, QUEUE, FULL, false, MULTI>(L);
 }
 
 // The full-wave tile of up to 4 words per lane, include_bits 1, rows without
@@ -625,7 +707,7 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
 template <int WPL, bool QUEUE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_num_sgpr(76))) void train_kernel_r4(tm_train_launch L)
 {
-  train_body<WPL, 64, 7, true, QUEUE, true, true>(L);
+  train_body<WPL, 64, 7, true, QUEUE, true, true, false>(L);
 }
 
 } // namespace
