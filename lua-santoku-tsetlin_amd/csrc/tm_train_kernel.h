@@ -190,6 +190,7 @@ template <typename T> using gptr = const __attribute__((address_space(1))) T *;
 template <typename T> __device__ __forceinline__ gptr<T> global_ptr(const T *p) { return (gptr<T>) p; }
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 
 template <int WPL>
 __device__ __forceinline__ void load_words(const uint32_t *row_generic, uint32_t wbase, uint32_t (&x)[WPL])
@@ -297,7 +298,7 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
   uint32_t *counts = P.counts;
   uint32_t *actions = P.actions;
 
-  __shared__ int32_t votes_lds[2][8];
+  __shared__ __attribute__((aligned(16))) int32_t votes_lds[2][8];
   // bits of the literals hit by random-literal draws, per clause; with
   // MULTI, on three levels: hit once (0), a second time (1), a third (2)
   constexpr int HL = MULTI ? 3 : 1;
@@ -343,7 +344,9 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
   const uint32_t T2 = 2 * T;
   // lanes past the row's end read its last tile instead (masked by every use)
   const uint32_t wload = FULL ? wbase : min(wbase, P.row_stride - WPL);
-  uint32_t buf = 0;
+  // byte offsets into votes_lds of this step's slot for clause j and of the
+  // step's first slot; both alternate between the two halves
+  uint32_t vw = j * 4, vr = 0;
   uint32_t n_steps = 0;
   __shared__ unsigned long long ev_lds[CNT_LDS ? 8 : 1][5];
   event_counters<G, CNT_LDS> cnt;
@@ -435,7 +438,7 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
     bool nq_t = false;
     auto take = [&]() {
       const int q = __builtin_ctzll(want);
-      want &= want - 1;
+      want &= ~(1ull << q);
       if constexpr (QUEUE) {
         const uint32_t v = __builtin_amdgcn_readlane(bs, q);
         nq_s = v & 0x7FFFFFFFu;
@@ -474,8 +477,13 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
         asm volatile("" : "+v"(x[k]));
       }
       // find the next sample and start loading its row before this one's work
-      const bool more = want ? true : scan();
-      if (more) {
+      // (the common case, a block with samples left, takes the next one
+      // without a branch on the scan's result)
+      bool more = true;
+      if (want) {
+        take();
+        fetch(nq_s);
+      } else if ((more = scan())) {
         take();
         fetch(nq_s);
       }
@@ -488,13 +496,15 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
       if (votes < 0) votes = 0;
       // a full-wave group's vote is the same on every lane: all lanes store
       // it (no exec-mask branch); smaller groups store from their lane 0
-      if (G == 64 || g == 0) votes_lds[buf][j] = (votes ^ vneg) - vneg;
+      char *const vbase = reinterpret_cast<char *>(&votes_lds[0][0]);
+      if (G == 64 || g == 0) *reinterpret_cast<int32_t *>(vbase + vw) = (votes ^ vneg) - vneg;
       lds_barrier();
-      int32_t v = 0;
-#pragma unroll
-      for (int c = 0; c < 8; c++) v += votes_lds[buf][c];
-      v = __builtin_amdgcn_readfirstlane(v);
-      buf ^= 1;
+      // the step's eight votes as two 16-byte reads (vr is a multiple of 32)
+      const i32x4 va = *reinterpret_cast<const i32x4 *>(vbase + vr);
+      const i32x4 vc = *reinterpret_cast<const i32x4 *>(vbase + vr + 16);
+      int32_t v = __builtin_amdgcn_readfirstlane(va.x + va.y + va.z + va.w + vc.x + vc.y + vc.z + vc.w);
+      vw ^= 32;
+      vr ^= 32;
       if (v > (int32_t) T) v = (int32_t) T;
       if (v < -(int32_t) T) v = -(int32_t) T;
       // fire iff u16 * 2T < (T -+ v) * 2^16; for integers x < num * 2^16
