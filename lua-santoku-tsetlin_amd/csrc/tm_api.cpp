@@ -90,11 +90,11 @@ uint32_t live_mask(uint32_t wd, uint32_t input_chunks, uint8_t tail_mask)
   return m;
 }
 
-// A trial whose rows are wd words with no bit of the last word masked: the
-// train kernel's variant for such rows keeps no masks.
+// A trial whose rows are wd words apart, wd words long, with no bit of the
+// last word masked: the train kernel's variant for such rows keeps no masks.
 uint32_t full_rows(const tm_train_trial &Q, uint32_t wd)
 {
-  return Q.wd == wd && Q.last_live == 0xFFFFFFFFu && Q.last_eval == 0xFFFFFFFFu;
+  return Q.wd == wd && Q.row_stride == wd && Q.last_live == 0xFFFFFFFFu && Q.last_eval == 0xFFFFFFFFu;
 }
 
 } // namespace

@@ -52,7 +52,7 @@ struct tm_train_launch {
   uint32_t init;                  // 1: start from the setup state (capi.c:1136-1146)
   uint32_t lanes, wpl;            // tile: lanes per clause, words per lane (0: default shape)
   uint32_t all_ib1;               // every trial has include_bits == 1
-  uint32_t all_full;              // every trial has width wd and all-ones last-word masks
+  uint32_t all_full;              // every trial has width and row stride wd and all-ones last-word masks
   // Step queue (classifier, one machine, one epoch per launch; NULL qid: the
   // kernel scans the epoch order itself).  Chunk ch trains on entries
   // [qoff[ch * qseg], qoff[(ch + 1) * qseg]) in order.

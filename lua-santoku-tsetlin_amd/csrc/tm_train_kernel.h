@@ -290,8 +290,8 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
   const uint32_t wbase = g * WPL;
   const uint32_t wd = P.wd;
   // Mask of word k of this lane.  FULL: every trial's row fills the tile
-  // exactly (G * WPL words) and both last-word masks are all ones, so no word
-  // is masked and the masks take no registers.
+  // exactly (G * WPL words, also the stride between rows) and both last-word
+  // masks are all ones, so no word is masked and the masks take no registers.
   auto wmask = [&](int k, uint32_t last) -> uint32_t {
     return FULL ? 0xFFFFFFFFu : word_mask(wbase + k, wd, last);
   };
@@ -458,7 +458,11 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
       nq_d = (dsel >> (16 * (j & 1))) & 0xFFFFu;
     };
     uint32_t xn[WPL];
-    auto fetch = [&](uint32_t sample) { load_words<WPL>(P.rows + (size_t) sample * P.row_stride, wload, xn); };
+    // FULL rows are G * WPL words apart (a power of two): a shift, not a multiply
+    auto fetch = [&](uint32_t sample) {
+      const size_t off = FULL ? (size_t) sample * (G * WPL) : (size_t) sample * P.row_stride;
+      load_words<WPL>(P.rows + off, wload, xn);
+    };
 
     if (!scan()) continue;
     take();
