@@ -179,9 +179,6 @@ __device__ __forceinline__ uint32_t included(const uint32_t (&st)[M + 1][WPL], i
   return v;
 }
 
-// Load WPL consecutive words of a literal row from word wbase.  Callers
-// clamp wbase into the row (the row stride is a multiple of WPL) so the load
-// is unconditional; words past the row's end are masked by every use.
 // Global-memory pointers.  Pointers that reach the kernel inside a struct
 // lose their address space, and loads through them become FLAT loads, which
 // count against lgkmcnt as well as vmcnt: the LDS-only barrier would then wait
@@ -192,6 +189,9 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 
+// Load WPL consecutive words of a literal row from word wbase.  Callers
+// clamp wbase into the row (the row stride is a multiple of WPL) so the load
+// is unconditional; words past the row's end are masked by every use.
 template <int WPL>
 __device__ __forceinline__ void load_words(const uint32_t *row_generic, uint32_t wbase, uint32_t (&x)[WPL])
 {
