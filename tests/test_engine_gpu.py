@@ -139,6 +139,37 @@ def test_train_workload1_bitexact():
     assert np.array_equal(eng.predict(d["test_rows"], 2048), orc.predict(d["test_rows"]))
 
 
+@pytest.mark.parametrize("features,kind,state", [(6, "classifier", 8), (40, "encoder", 8), (300, "classifier", 8),
+                                                 (24, "classifier", 3), (1024, "classifier", 8)])
+def test_repeat_hits_bitexact(features, kind, state):
+    """Specificity 1: an event draws as many random literals as there are
+    literals, so literals are hit two, three and four or more times by one
+    event (Poisson(1)), often while their counters are at 1 or 2.  Covers
+    the clamped multi-level subtract of the random-literal draws, its
+    saturation at zero, the rounds left for a fourth hit, and the
+    round-per-hit path of the four-workgroup tile (1024 features)."""
+    rng = np.random.default_rng(features + state)
+    n = 160
+    o = dict(classes=4, features=features, clauses=16, clause_tolerance=6, clause_maximum=8, specificity=1.0,
+             target=6, seed=4321, state=state)
+    eng = tm.create(kind, o)
+    orc = O.Machine(O.params(kind, **o))
+    rows = rows_for(features, n, rng, 0.3)
+    if kind == "classifier":
+        labels = rng.integers(0, 4, n).astype(np.int64)
+        eng.train(samples=n, problems=rows, solutions=labels, iterations=3)
+        orc.train_replay(rows, labels=labels, epochs=3)
+    else:
+        codes = rng.integers(0, 256, (n, 1), dtype=np.uint8)
+        eng.train(samples=n, sentences=rows, codes=codes, iterations=3)
+        orc.train_replay(rows, codes=codes, epochs=3)
+    # oracle counters: steps, Type I true, Type II, Type I false, draws, skips
+    assert orc.stats[3] > 0 and orc.stats[4] == orc.stats[3] * 2 * features
+    assert np.array_equal(eng.checkpoint(), orc.actions)
+    st = eng.train_stats()
+    assert st["type1_false"] == int(orc.stats[3]) and st["draws"] == int(orc.stats[4])
+
+
 def test_negative_sampling_variants():
     rng = np.random.default_rng(5)
     rows = rows_for(40, 300, rng)
