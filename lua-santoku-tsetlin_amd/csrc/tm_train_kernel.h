@@ -566,8 +566,20 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
                 const uint32_t cnt = min(32u, nd - base);
                 uint32_t r = base * G + g, hin = kev + r * TM_GOLDEN;
                 const uint32_t hstep = G * TM_GOLDEN;
+                // every lane's draw exists in the first `full` iterations
+                // (base * G < s_draws); only the batch's last may be short
+                const uint32_t full = min(cnt, (P.s_draws - base * G) / G);
+                uint32_t i = 0;
 #pragma unroll 4
-                for (uint32_t i = 0; i < cnt; i++) {
+                for (; i < full; i++) {
+                  const uint32_t lbit = tm_mulhi(tm_lowbias32(hin), P.input_bits);
+                  const uint32_t hb = 1u << (lbit & 31);
+                  const uint32_t old = atomicOr(&hits[0][j][lbit >> 5], hb);
+                  if (old & hb) pend |= 1u << i;
+                  hin += hstep;
+                }
+                r += full * G;
+                for (; i < cnt; i++) {
                   const bool valid = r < P.s_draws;
                   const uint32_t lbit = tm_mulhi(tm_lowbias32(hin), P.input_bits);
                   const uint32_t hb = valid ? 1u << (lbit & 31) : 0u;
