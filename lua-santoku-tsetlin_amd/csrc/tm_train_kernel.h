@@ -570,7 +570,19 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
                 // (base * G < s_draws); only the batch's last may be short
                 const uint32_t full = min(cnt, (P.s_draws - base * G) / G);
                 uint32_t i = 0;
-#pragma unroll 4
+                // four draws at a time: their hashes first, then their four
+                // atomics in flight together, then the hits they found
+                for (; i + 4 <= full; i += 4) {
+                  uint32_t lb[4], old[4];
+#pragma unroll
+                  for (int u = 0; u < 4; u++) lb[u] = tm_mulhi(tm_lowbias32(hin + u * hstep), P.input_bits);
+#pragma unroll
+                  for (int u = 0; u < 4; u++) old[u] = atomicOr(&hits[0][j][lb[u] >> 5], 1u << (lb[u] & 31));
+#pragma unroll
+                  for (int u = 0; u < 4; u++)
+                    if (old[u] & (1u << (lb[u] & 31))) pend |= 1u << (i + u);
+                  hin += 4 * hstep;
+                }
                 for (; i < full; i++) {
                   const uint32_t lbit = tm_mulhi(tm_lowbias32(hin), P.input_bits);
                   const uint32_t hb = 1u << (lbit & 31);
