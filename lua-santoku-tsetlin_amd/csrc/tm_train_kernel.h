@@ -3,6 +3,8 @@
 // Included by tm_kernels.hip (8 state bits, every mode) and by
 // tm_train_states.hip (other state bits, one object per plane count).
 #pragma once
+#include <type_traits>
+
 #include "tm_internal.h"
 #include "tm_rng.h"
 
@@ -629,26 +631,39 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
                 }
               }
               __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
+              if constexpr (MULTI) {
+                // levels nest, so a literal was hit h0 + h1 + h2 times: a
+                // two-bit count (k1 k0), clamped to the counter (a counter
+                // below 4 is the two bits c1 c0) so that the subtraction
+                // saturates at zero.  Each level is read and cleared in one
+                // exchange per word; the level count is uniform, so the
+                // branch on it stays outside the words.
+                auto count = [&](auto nlev) {
+                  constexpr uint32_t NL = decltype(nlev)::value;
 #pragma unroll
-              for (int k = 0; k < WPL; k++) {   // read and clear in one exchange
-                if constexpr (MULTI) {
-                  // levels nest, so a literal was hit h0 + h1 + h2 times: a
-                  // two-bit count (k1 k0), clamped to the counter (a counter
-                  // below 4 is the two bits c1 c0) so that the subtraction
-                  // saturates at zero
-                  const uint32_t h0 = atomicExch(&hits[0][j][wbase + k], 0u);
-                  const uint32_t h1 = levels > 1 ? atomicExch(&hits[1][j][wbase + k], 0u) : 0u;
-                  const uint32_t h2 = levels > 2 ? atomicExch(&hits[2][j][wbase + k], 0u) : 0u;
-                  const uint32_t k0 = h0 ^ h1 ^ h2, k1 = h1;
-                  const uint32_t c0 = st[0][k], c1 = st[1][k];
-                  const uint32_t hi = M > 1 ? any_one_from<M, WPL, 2>(st, k) : 0u;
-                  // count above counter: k1 > c1, or k1 == c1 and k0 > c0
-                  const uint32_t gt = ~hi & ((k1 & ~c1) | (~(k1 ^ c1) & k0 & ~c0));
-                  mk[k] = (gt & c0) | (~gt & k0);
-                  m1[k] = (gt & c1) | (~gt & k1);
-                } else {
-                  mk[k] = atomicExch(&hits[0][j][wbase + k], 0u) & any_one<M, WPL, M>(st, k);
-                }
+                  for (int k = 0; k < WPL; k++) {
+                    const uint32_t h0 = atomicExch(&hits[0][j][wbase + k], 0u);
+                    const uint32_t h1 = NL > 1 ? atomicExch(&hits[NL > 1 ? 1 : 0][j][wbase + k], 0u) : 0u;
+                    const uint32_t h2 = NL > 2 ? atomicExch(&hits[NL > 2 ? 2 : 0][j][wbase + k], 0u) : 0u;
+                    const uint32_t k0 = h0 ^ h1 ^ h2, k1 = h1;
+                    const uint32_t c0 = st[0][k], c1 = st[1][k];
+                    const uint32_t hi = M > 1 ? any_one_from<M, WPL, 2>(st, k) : 0u;
+                    // count above counter: k1 > c1, or k1 == c1 and k0 > c0
+                    const uint32_t gt = ~hi & ((k1 & ~c1) | (~(k1 ^ c1) & k0 & ~c0));
+                    mk[k] = (gt & c0) | (~gt & k0);
+                    m1[k] = (gt & c1) | (~gt & k1);
+                  }
+                };
+                if (levels > 2) count(std::integral_constant<uint32_t, 3>());
+                else if (levels > 1) count(std::integral_constant<uint32_t, 2>());
+                else count(std::integral_constant<uint32_t, 1>());
+              } else {
+                // read and clear every word first, so the exchanges are in
+                // flight together
+#pragma unroll
+                for (int k = 0; k < WPL; k++) mk[k] = atomicExch(&hits[0][j][wbase + k], 0u);
+#pragma unroll
+                for (int k = 0; k < WPL; k++) mk[k] &= any_one<M, WPL, M>(st, k);
               }
               __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
               sgn = 0u;
