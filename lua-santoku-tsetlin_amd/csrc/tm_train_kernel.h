@@ -573,18 +573,35 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
                 const uint32_t full = min(cnt, (P.s_draws - base * G) / G);
                 uint32_t i = 0;
                 // four draws at a time: their hashes first, then their four
-                // atomics in flight together, then the hits they found
-                for (; i + 4 <= full; i += 4) {
-                  uint32_t lb[4], old[4];
+                // atomics in flight together, then the hits they found.  With
+                // input_bits = 2^b the draw mulhi(h, 2^b) is h >> (32 - b),
+                // a full-rate shift instead of a quarter-rate multiply, and
+                // the repeat test is one bit extract.  The four-workgroup
+                // tile (CNT_LDS) keeps the multiply and the compare: a second
+                // copy of the loop or the extract take it past 64 VGPRs.
+                constexpr bool LEAN = !CNT_LDS;
+                auto batch = [&](auto pow2) {
+                  constexpr bool POW2 = decltype(pow2)::value;
+                  const uint32_t sh = POW2 ? __builtin_clz(P.input_bits) + 1 : 0u;
+                  for (; i + 4 <= full; i += 4) {
+                    uint32_t lb[4], old[4];
 #pragma unroll
-                  for (int u = 0; u < 4; u++) lb[u] = tm_mulhi(tm_lowbias32(hin + u * hstep), P.input_bits);
+                    for (int u = 0; u < 4; u++) {
+                      const uint32_t h = tm_lowbias32(hin + u * hstep);
+                      lb[u] = POW2 ? h >> sh : tm_mulhi(h, P.input_bits);
+                    }
 #pragma unroll
-                  for (int u = 0; u < 4; u++) old[u] = atomicOr(&hits[0][j][lb[u] >> 5], 1u << (lb[u] & 31));
+                    for (int u = 0; u < 4; u++) old[u] = atomicOr(&hits[0][j][lb[u] >> 5], 1u << (lb[u] & 31));
 #pragma unroll
-                  for (int u = 0; u < 4; u++)
-                    if (old[u] & (1u << (lb[u] & 31))) pend |= 1u << (i + u);
-                  hin += 4 * hstep;
-                }
+                    for (int u = 0; u < 4; u++) {
+                      if constexpr (LEAN) pend |= __builtin_amdgcn_ubfe(old[u], lb[u] & 31, 1) << (i + u);
+                      else if (old[u] & (1u << (lb[u] & 31))) pend |= 1u << (i + u);
+                    }
+                    hin += 4 * hstep;
+                  }
+                };
+                if (LEAN && (P.input_bits & (P.input_bits - 1)) == 0) batch(std::integral_constant<bool, LEAN>());
+                else batch(std::false_type());
                 for (; i < full; i++) {
                   const uint32_t lbit = tm_mulhi(tm_lowbias32(hin), P.input_bits);
                   const uint32_t hb = 1u << (lbit & 31);
