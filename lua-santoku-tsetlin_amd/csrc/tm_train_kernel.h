@@ -772,8 +772,9 @@ __global__ __launch_bounds__(8 * G, (WPL == 8 && IB1 ? 4 : 1)) void train_kernel
 // takes at most 64 VGPRs and few enough SGPRs: a CU of gfx950 holds four
 // 512-thread workgroups of 64 VGPRs at 70 SGPRs each but only three at 86, so
 // the kernel is held to 76 SGPRs (the compiler keeps the rest in the lanes of
-// a VGPR) and its event counters live in LDS.  The 4-word tile then takes 63
-// VGPRs by itself; a 64-VGPR launch bound instead makes the compiler spill.
+// a VGPR) and its event counters live in LDS.  The 4-word tile then takes 64
+// VGPRs by itself; an occupancy bound of 8 waves per SIMD instead makes the
+// compiler spill whenever the body needs more.
 template <int WPL, bool QUEUE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_num_sgpr(76))) void train_kernel_r4(tm_train_launch L)
 {
