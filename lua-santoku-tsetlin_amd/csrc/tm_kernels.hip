@@ -280,33 +280,60 @@ __global__ __launch_bounds__(256) void queue_count_kernel(tm_queue_launch Q)
 
 // Same grid: each wave writes its segment's entries from its offset on,
 // in epoch order, with the step draws of tm_philox(sample, chunk, epoch).
+// About one position in ten is an entry, so the wave compacts its entries
+// into LDS and computes the draws 64 at a time with every lane busy (Philox
+// in the branch of the wanted lanes would run for nearly every 64 positions).
+__device__ __forceinline__ void queue_emit(const tm_queue_launch &Q, uint32_t e, uint64_t idx, uint32_t ch)
+{
+  Q.qid[idx] = e;
+  const tm_u4 d = tm_philox(e & 0x7FFFFFFFu, ch, Q.epoch, TM_TAG_STEP, Q.seed_lo, Q.seed_hi);
+  Q.qdraw[idx] = d.x;
+  Q.qdraw[Q.qcap + idx] = d.y;
+  Q.qdraw[2 * Q.qcap + idx] = d.z;
+  Q.qdraw[3 * Q.qcap + idx] = d.w;
+}
+
 __global__ __launch_bounds__(256) void queue_write_kernel(tm_queue_launch Q)
 {
+  __shared__ uint32_t pend_lds[4][128];
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t seg = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t seg = blockIdx.y * 4 + wv;
   const uint32_t ch = blockIdx.x;
   if (seg >= Q.nseg) return;
+  uint32_t *pend = pend_lds[wv];
   const uint32_t cls = ch / Q.clause_chunks;
   const uint32_t p0 = seg * kQueueSeg;
   const uint32_t p1 = (uint32_t) min((uint64_t) Q.n, (uint64_t) p0 + kQueueSeg);
-  uint64_t base = Q.offsets[(size_t) ch * Q.nseg + seg];
+  uint64_t base = Q.offsets[(size_t) ch * Q.nseg + seg];   // output index of pend[0]
   const uint64_t below = (1ull << lane) - 1;
+  uint32_t np = 0;                                          // entries waiting in pend (< 64 between rounds)
   for (uint32_t b = p0; b < p1; b += 64) {
     uint32_t s;
     bool tgt;
     const bool w = queue_wanted(Q, b + lane, ch, cls, s, tgt);
     const uint64_t mask = __ballot(w);
-    if (w) {
-      const uint64_t idx = base + __popcll(mask & below);
-      Q.qid[idx] = s | (tgt ? 0x80000000u : 0u);
-      const tm_u4 d = tm_philox(s, ch, Q.epoch, TM_TAG_STEP, Q.seed_lo, Q.seed_hi);
-      Q.qdraw[idx] = d.x;
-      Q.qdraw[Q.qcap + idx] = d.y;
-      Q.qdraw[2 * Q.qcap + idx] = d.z;
-      Q.qdraw[3 * Q.qcap + idx] = d.w;
+    if (w) pend[np + __popcll(mask & below)] = s | (tgt ? 0x80000000u : 0u);
+    np += __popcll(mask);
+    if (np >= 64) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+      const uint32_t e = pend[lane];
+      const uint32_t rest = lane + 64 < np ? pend[lane + 64] : 0u;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+      if (lane + 64 < np) pend[lane] = rest;
+      queue_emit(Q, e, base + lane, ch);
+      base += 64;
+      np -= 64;
     }
-    base += __popcll(mask);
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+  if (lane < np) queue_emit(Q, pend[lane], base + lane, ch);
 }
 
 template <int WPL, int G>
