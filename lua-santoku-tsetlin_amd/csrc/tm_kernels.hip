@@ -246,43 +246,56 @@ __global__ __launch_bounds__(256) void queue_keys_kernel(tm_queue_launch Q)
 }
 
 // tm_skip_draw(ek0, ek1, s, ch) from the sample's precomputed inner hash
-__device__ __forceinline__ bool queue_wanted(const tm_queue_launch &Q, uint32_t pos, uint32_t ch, uint32_t cls,
-                                             uint32_t &s, bool &tgt)
+// key; sc is the sample's class.  An entry: a sample of the chunk's class, or
+// a negative that the skip draw keeps.
+__device__ __forceinline__ bool queue_keep(const tm_queue_launch &Q, bool valid, uint32_t key, uint32_t sc,
+                                           uint32_t ch, bool &tgt)
 {
-  const bool valid = pos < Q.n;
-  s = valid ? Q.perm[pos] : 0u;
-  const uint32_t key = valid ? Q.pos_keys[pos] : 0u;
-  const uint32_t sc = valid ? Q.pos_keys[Q.n + pos] : 0xFFFFFFFFu;
-  tgt = valid && sc == cls;
+  tgt = valid && sc == ch / Q.clause_chunks;
   return valid && (tgt || (uint64_t) tm_lowbias32(key ^ (ch * TM_GOLDEN + Q.ek1)) >= Q.skip_threshold);
 }
 
-// grid (chunks, ceil(nseg / 4)), 4 waves per block, one (chunk, segment) each
+// Chunks per wave of the two queue passes: the wave reads a position's key
+// and class once for all of them.
+constexpr uint32_t kQueueCpw = 4;
+
+// grid (ceil(chunks / kQueueCpw), ceil(nseg / 4)), 4 waves per block, each
+// wave one segment of up to kQueueCpw consecutive chunks
 __global__ __launch_bounds__(256) void queue_count_kernel(tm_queue_launch Q)
 {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t seg = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const uint32_t ch = blockIdx.x;
+  const uint32_t c0 = blockIdx.x * kQueueCpw;
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) Q.counts[(size_t) Q.chunks * Q.nseg] = 0;
   if (seg >= Q.nseg) return;
-  const uint32_t cls = ch / Q.clause_chunks;
   const uint32_t p0 = seg * kQueueSeg;
   const uint32_t p1 = (uint32_t) min((uint64_t) Q.n, (uint64_t) p0 + kQueueSeg);
-  uint64_t cnt = 0;
+  uint64_t cnt[kQueueCpw] = {};
   for (uint32_t b = p0; b < p1; b += 64) {
-    uint32_t s;
-    bool tgt;
-    const bool w = queue_wanted(Q, b + lane, ch, cls, s, tgt);
-    cnt += __popcll(__ballot(w));
+    const uint32_t pos = b + lane;
+    const bool valid = pos < Q.n;
+    const uint32_t key = valid ? Q.pos_keys[pos] : 0u;
+    const uint32_t sc = valid ? Q.pos_keys[Q.n + pos] : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t k = 0; k < kQueueCpw; k++) {
+      if (c0 + k >= Q.chunks) break;
+      bool tgt;
+      cnt[k] += __popcll(__ballot(queue_keep(Q, valid, key, sc, c0 + k, tgt)));
+    }
   }
-  if (lane == 0) Q.counts[(size_t) ch * Q.nseg + seg] = cnt;
+  if (lane == 0) {
+#pragma unroll
+    for (uint32_t k = 0; k < kQueueCpw; k++)
+      if (c0 + k < Q.chunks) Q.counts[(size_t)(c0 + k) * Q.nseg + seg] = cnt[k];
+  }
 }
 
-// Same grid: each wave writes its segment's entries from its offset on,
-// in epoch order, with the step draws of tm_philox(sample, chunk, epoch).
-// About one position in ten is an entry, so the wave compacts its entries
-// into LDS and computes the draws 64 at a time with every lane busy (Philox
-// in the branch of the wanted lanes would run for nearly every 64 positions).
+// Same grid: each wave writes each of its chunks' entries of its segment
+// from their offsets on, in epoch order, with the step draws of
+// tm_philox(sample, chunk, epoch).  About one position in ten is an entry, so
+// the wave compacts each chunk's entries into LDS and computes the draws 64
+// at a time with every lane busy (Philox in the branch of the wanted lanes
+// would run for nearly every 64 positions).
 __device__ __forceinline__ void queue_emit(const tm_queue_launch &Q, uint32_t e, uint64_t idx, uint32_t ch)
 {
   Q.qid[idx] = e;
@@ -293,47 +306,62 @@ __device__ __forceinline__ void queue_emit(const tm_queue_launch &Q, uint32_t e,
   Q.qdraw[3 * Q.qcap + idx] = d.w;
 }
 
-__global__ __launch_bounds__(256) void queue_write_kernel(tm_queue_launch Q)
+__device__ __forceinline__ void wave_lds_sync()
 {
-  __shared__ uint32_t pend_lds[4][128];
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wv = threadIdx.x >> 6;
-  const uint32_t seg = blockIdx.y * 4 + wv;
-  const uint32_t ch = blockIdx.x;
-  if (seg >= Q.nseg) return;
-  uint32_t *pend = pend_lds[wv];
-  const uint32_t cls = ch / Q.clause_chunks;
-  const uint32_t p0 = seg * kQueueSeg;
-  const uint32_t p1 = (uint32_t) min((uint64_t) Q.n, (uint64_t) p0 + kQueueSeg);
-  uint64_t base = Q.offsets[(size_t) ch * Q.nseg + seg];   // output index of pend[0]
-  const uint64_t below = (1ull << lane) - 1;
-  uint32_t np = 0;                                          // entries waiting in pend (< 64 between rounds)
-  for (uint32_t b = p0; b < p1; b += 64) {
-    uint32_t s;
-    bool tgt;
-    const bool w = queue_wanted(Q, b + lane, ch, cls, s, tgt);
-    const uint64_t mask = __ballot(w);
-    if (w) pend[np + __popcll(mask & below)] = s | (tgt ? 0x80000000u : 0u);
-    np += __popcll(mask);
-    if (np >= 64) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-      const uint32_t e = pend[lane];
-      const uint32_t rest = lane + 64 < np ? pend[lane + 64] : 0u;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-      if (lane + 64 < np) pend[lane] = rest;
-      queue_emit(Q, e, base + lane, ch);
-      base += 64;
-      np -= 64;
-    }
-  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
-  if (lane < np) queue_emit(Q, pend[lane], base + lane, ch);
+}
+
+__global__ __launch_bounds__(256) void queue_write_kernel(tm_queue_launch Q)
+{
+  __shared__ uint32_t pend_lds[4][kQueueCpw][128];
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wv = threadIdx.x >> 6;
+  const uint32_t seg = blockIdx.y * 4 + wv;
+  const uint32_t c0 = blockIdx.x * kQueueCpw;
+  if (seg >= Q.nseg) return;
+  const uint32_t p0 = seg * kQueueSeg;
+  const uint32_t p1 = (uint32_t) min((uint64_t) Q.n, (uint64_t) p0 + kQueueSeg);
+  const uint64_t below = (1ull << lane) - 1;
+  uint64_t base[kQueueCpw];   // output index of each chunk's pend[0]
+  uint32_t np[kQueueCpw];     // entries waiting per chunk (< 64 between rounds)
+#pragma unroll
+  for (uint32_t k = 0; k < kQueueCpw; k++) {
+    base[k] = c0 + k < Q.chunks ? Q.offsets[(size_t)(c0 + k) * Q.nseg + seg] : 0;
+    np[k] = 0;
+  }
+  for (uint32_t b = p0; b < p1; b += 64) {
+    const uint32_t pos = b + lane;
+    const bool valid = pos < Q.n;
+    const uint32_t s = valid ? Q.perm[pos] : 0u;
+    const uint32_t key = valid ? Q.pos_keys[pos] : 0u;
+    const uint32_t sc = valid ? Q.pos_keys[Q.n + pos] : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t k = 0; k < kQueueCpw; k++) {
+      if (c0 + k >= Q.chunks) break;
+      uint32_t *pend = pend_lds[wv][k];
+      bool tgt;
+      const bool w = queue_keep(Q, valid, key, sc, c0 + k, tgt);
+      const uint64_t mask = __ballot(w);
+      if (w) pend[np[k] + __popcll(mask & below)] = s | (tgt ? 0x80000000u : 0u);
+      np[k] += __popcll(mask);
+      if (np[k] >= 64) {
+        wave_lds_sync();
+        const uint32_t e = pend[lane];
+        const uint32_t rest = lane + 64 < np[k] ? pend[lane + 64] : 0u;
+        wave_lds_sync();
+        if (lane + 64 < np[k]) pend[lane] = rest;
+        queue_emit(Q, e, base[k] + lane, c0 + k);
+        base[k] += 64;
+        np[k] -= 64;
+      }
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (uint32_t k = 0; k < kQueueCpw; k++)
+    if (c0 + k < Q.chunks && lane < np[k]) queue_emit(Q, pend_lds[wv][k][lane], base[k] + lane, c0 + k);
 }
 
 template <int WPL, int G>
@@ -523,7 +551,8 @@ int tm_launch_queue_count(const tm_queue_launch &Q, void *scratch, size_t scratc
       (Q.nseg + 3) / 4 > 65535)
     return (int) hipErrorInvalidValue;
   hipLaunchKernelGGL(queue_keys_kernel, dim3((Q.n + 255) / 256), dim3(256), 0, st, Q);
-  hipLaunchKernelGGL(queue_count_kernel, dim3(Q.chunks, (Q.nseg + 3) / 4), dim3(256), 0, st, Q);
+  hipLaunchKernelGGL(queue_count_kernel, dim3((Q.chunks + kQueueCpw - 1) / kQueueCpw, (Q.nseg + 3) / 4), dim3(256), 0,
+                     st, Q);
   size_t tb = scratch_bytes;
   hipError_t err = hipcub::DeviceScan::ExclusiveSum(scratch, tb, Q.counts, Q.offsets, (int) items, st);
   if (err != hipSuccess) return (int) err;
@@ -534,7 +563,8 @@ int tm_launch_queue_write(const tm_queue_launch &Q, hipStream_t st)
 {
   if (!Q.qid || !Q.qdraw || !Q.pos_keys || Q.nseg != tm_queue_segments(Q.n) || (Q.nseg + 3) / 4 > 65535)
     return (int) hipErrorInvalidValue;
-  hipLaunchKernelGGL(queue_write_kernel, dim3(Q.chunks, (Q.nseg + 3) / 4), dim3(256), 0, st, Q);
+  hipLaunchKernelGGL(queue_write_kernel, dim3((Q.chunks + kQueueCpw - 1) / kQueueCpw, (Q.nseg + 3) / 4), dim3(256), 0,
+                     st, Q);
   return (int) hipGetLastError();
 }
 
