@@ -257,7 +257,7 @@ __device__ __forceinline__ bool queue_keep(const tm_queue_launch &Q, bool valid,
 
 // Chunks per wave of the two queue passes: the wave reads a position's key
 // and class once for all of them.
-constexpr uint32_t kQueueCpw = 4;
+constexpr uint32_t kQueueCpw = 8;
 
 // grid (ceil(chunks / kQueueCpw), ceil(nseg / 4)), 4 waves per block, each
 // wave one segment of up to kQueueCpw consecutive chunks
