@@ -547,6 +547,10 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
             // for the next round.
             kev = tm_literal_key(ek2, ek3, sq, clause_r);
             nd = (P.s_draws + G - 1) / G;
+            // this clause is usually the one the chunk's next vote waits
+            // for: its waves issue ahead of the CU's other waves until its
+            // decrements are applied
+            __builtin_amdgcn_s_setprio(2);
             cnt.add(3, 1);
             cnt.add(4, P.s_draws);
           }
@@ -735,6 +739,7 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
             lit = recount();
             cap = cap_of(lit);
           }
+          if (kind == 2) __builtin_amdgcn_s_setprio(0);
         }
       }
       if (!more) break;
