@@ -482,6 +482,11 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
         x[k] = xn[k];
         asm volatile("" : "+v"(x[k]));
       }
+      // On the four-workgroup tile (CNT_LDS) a step's evaluation, up to the
+      // vote barrier, issues ahead of other workgroups' feedback at priority
+      // 0: every chunk's next step waits on all eight evaluations.  With two
+      // workgroups per CU (the wider tiles) it measured slower.
+      if constexpr (CNT_LDS) __builtin_amdgcn_s_setprio(1);
       // find the next sample and start loading its row before this one's work
       // (the common case, a block with samples left, takes the next one
       // without a branch on the scan's result)
@@ -505,6 +510,7 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
       char *const vbase = reinterpret_cast<char *>(&votes_lds[0][0]);
       if (G == 64 || g == 0) *reinterpret_cast<int32_t *>(vbase + vw) = (votes ^ vneg) - vneg;
       lds_barrier();
+      if constexpr (CNT_LDS) __builtin_amdgcn_s_setprio(0);
       // the step's eight votes as two 16-byte reads (vr is a multiple of 32)
       const i32x4 va = *reinterpret_cast<const i32x4 *>(vbase + vr);
       const i32x4 vc = *reinterpret_cast<const i32x4 *>(vbase + vr + 16);
