@@ -482,11 +482,14 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
         x[k] = xn[k];
         asm volatile("" : "+v"(x[k]));
       }
-      // On the four-workgroup tile (CNT_LDS) a step's evaluation, up to the
-      // vote barrier, issues ahead of other workgroups' feedback at priority
-      // 0: every chunk's next step waits on all eight evaluations.  With two
-      // workgroups per CU (the wider tiles) it measured slower.
-      if constexpr (CNT_LDS) __builtin_amdgcn_s_setprio(1);
+      // Wave priorities (measured orderings): on the four-workgroup tile
+      // (CNT_LDS) a step's evaluation, up to the vote barrier, issues first
+      // (2), since every chunk's next step waits on all eight evaluations;
+      // then a clause applying random-literal decrements (1), usually the
+      // one its chunk waits for; then the rest (0).  With two workgroups per
+      // CU (the wider tiles) a prioritised evaluation measured slower, and
+      // only the random-literal clause is raised (2).
+      if constexpr (CNT_LDS) __builtin_amdgcn_s_setprio(2);
       // find the next sample and start loading its row before this one's work
       // (the common case, a block with samples left, takes the next one
       // without a branch on the scan's result)
@@ -553,10 +556,8 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
             // for the next round.
             kev = tm_literal_key(ek2, ek3, sq, clause_r);
             nd = (P.s_draws + G - 1) / G;
-            // this clause is usually the one the chunk's next vote waits
-            // for: its waves issue ahead of the CU's other waves until its
-            // decrements are applied
-            __builtin_amdgcn_s_setprio(2);
+            // raised until its decrements are applied (see the loop head)
+            __builtin_amdgcn_s_setprio(CNT_LDS ? 1 : 2);
             cnt.add(3, 1);
             cnt.add(4, P.s_draws);
           }
