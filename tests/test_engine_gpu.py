@@ -220,6 +220,25 @@ def test_step_queue_matches_scan(features, classes, clauses, neg, n, monkeypatch
     assert got["1"][1]["steps"] == int(orc.stats[0])
 
 
+@pytest.mark.parametrize("tolerance", [8191, 8192, 9000])
+def test_vote_width_at_large_tolerance(tolerance):
+    """Tolerances (and clause maxima) past the 16-bit range of four summed
+    votes, on the four-workgroup tile (2048 literals, a full 64-lane tile):
+    the votes, the chunk clamp and the fire test must keep the replay's
+    machine."""
+    rng = np.random.default_rng(tolerance)
+    n = 300
+    rows = rows_for(1024, n, rng, 0.3)
+    labels = rng.integers(0, 3, n).astype(np.int64)
+    o = dict(features=1024, classes=3, clauses=16, clause_tolerance=tolerance, clause_maximum=tolerance,
+             specificity=8.0, target=tolerance // 2, seed=17)
+    eng, orc = pair(**o)
+    eng.train(samples=n, problems=rows, solutions=labels, iterations=2)
+    orc.train_replay(rows, labels=labels, epochs=2)
+    assert np.array_equal(eng.checkpoint(), orc.actions)
+    assert eng.train_stats()["steps"] == int(orc.stats[0])
+
+
 @pytest.mark.parametrize("kind", ["classifier", "encoder"])
 def test_every_tile_gives_the_same_machine(kind, monkeypatch):
     """32 words per row: tiles 16x2, 32x1, 32x2, 64x1 and 64x2 (lanes x words)
