@@ -101,16 +101,31 @@ __global__ __launch_bounds__(64 * PRED_WAVES) void predict_kernel(tm_predict_lau
 #pragma unroll
   for (int i = 0; i < PRED_SPW; i++) acc[i] = 0;
 
+  // each thread stages STG words of a slice; the next slice's words are
+  // loaded while this one is evaluated (all-ones stands for a word past the
+  // rows, so its complement is zero)
+  // Thread tid stages column tid % PRED_WC of rows s0 + tid / PRED_WC + k * RS.
+  constexpr uint32_t STG = PRED_SPW * PRED_WC / 64, RS = 64 * PRED_WAVES / PRED_WC;
+  static_assert(PRED_WAVES * PRED_SPW * PRED_WC == STG * 64 * PRED_WAVES && (64 * PRED_WAVES) % PRED_WC == 0,
+                "slice staging");
+  const uint32_t srow = s0 + tid / PRED_WC, wcol = tid % PRED_WC;
+  const uint32_t *const rb = L.rows + (size_t) min(srow, L.n - 1) * L.wd;
+  uint32_t pre[STG];
+  auto fetch_slice = [&](uint32_t w0) {
+    const uint32_t w = w0 + wcol;
+#pragma unroll
+    for (uint32_t k = 0; k < STG; k++)
+      pre[k] = (srow + k * RS < L.n && w < L.wd) ? rb[k * RS * L.wd + w] : 0xFFFFFFFFu;
+  };
+  fetch_slice(0);
   for (uint32_t w0 = 0; w0 < L.wd; w0 += PRED_WC) {
     __syncthreads();
     uint32_t *xw = reinterpret_cast<uint32_t *>(&xs[0][0]);
-    for (uint32_t i = tid; i < PRED_WAVES * PRED_SPW * PRED_WC; i += 64 * PRED_WAVES) {
-      const uint32_t sl = i / PRED_WC, ww = i % PRED_WC, s = s0 + sl, w = w0 + ww;
-      uint32_t v = 0;
-      if (s < L.n && w < L.wd) v = ~L.rows[(size_t) s * L.wd + w] & word_mask(w, L.wd, L.last_eval);
-      xw[i] = v;
-    }
+    const uint32_t wm = word_mask(w0 + wcol, L.wd, L.last_eval);
+#pragma unroll
+    for (uint32_t k = 0; k < STG; k++) xw[tid + k * 64 * PRED_WAVES] = ~pre[k] & wm;
     __syncthreads();
+    if (w0 + PRED_WC < L.wd) fetch_slice(w0 + PRED_WC);
     uint32_t a[PRED_WC];
 #pragma unroll
     for (int ww = 0; ww < PRED_WC; ww++) {
