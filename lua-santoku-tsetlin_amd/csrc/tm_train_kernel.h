@@ -668,11 +668,19 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
                 // branch on it stays outside the words.
                 auto count = [&](auto nlev) {
                   constexpr uint32_t NL = decltype(nlev)::value;
+                  // every level's words are read and cleared first, so the
+                  // exchanges are in flight together
+                  uint32_t hl[NL][WPL];
+#pragma unroll
+                  for (uint32_t l = 0; l < NL; l++) {
+#pragma unroll
+                    for (int k = 0; k < WPL; k++) hl[l][k] = atomicExch(&hits[l][j][wbase + k], 0u);
+                  }
 #pragma unroll
                   for (int k = 0; k < WPL; k++) {
-                    const uint32_t h0 = atomicExch(&hits[0][j][wbase + k], 0u);
-                    const uint32_t h1 = NL > 1 ? atomicExch(&hits[NL > 1 ? 1 : 0][j][wbase + k], 0u) : 0u;
-                    const uint32_t h2 = NL > 2 ? atomicExch(&hits[NL > 2 ? 2 : 0][j][wbase + k], 0u) : 0u;
+                    const uint32_t h0 = hl[0][k];
+                    const uint32_t h1 = NL > 1 ? hl[NL > 1 ? 1 : 0][k] : 0u;
+                    const uint32_t h2 = NL > 2 ? hl[NL > 2 ? 2 : 0][k] : 0u;
                     const uint32_t k0 = h0 ^ h1 ^ h2, k1 = h1;
                     const uint32_t c0 = st[0][k], c1 = st[1][k];
                     const uint32_t hi = M > 1 ? any_one_from<M, WPL, 2>(st, k) : 0u;
