@@ -1,12 +1,8 @@
 """Build libtsetlin_amd.so in place (hipcc, gfx950 only).
 
-    python lua-santoku-tsetlin_amd/build.py            # the library
-    python lua-santoku-tsetlin_amd/build.py --probe    # + probe_valu_issue, probe_residency
+    python lua-santoku-tsetlin_amd/build.py
 
 Objects go to lua-santoku-tsetlin_amd/_build/, the library next to this file.
-probe_valu_issue (next to this file too) measures the integer VALU issue rate
-that prices the train kernel's roofline; probe_residency measures how many
-train-kernel-shaped workgroups a CU holds for a given register footprint.
 """
 import os
 import subprocess
@@ -62,24 +58,6 @@ def build(verbose=False):
     return OUT
 
 
-def build_probe(verbose=False):
-    outs = []
-    for name in ("probe_valu_issue", "probe_residency"):
-        out = os.path.join(PKG, name)
-        cmd = [HIPCC, "-O3", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-               "-Wno-unused-command-line-argument", "-Wno-unused-result", os.path.join(CSRC, name + ".hip"),
-               "-o", out]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (name, " ".join(cmd), r.stderr[-4000:]))
-        if verbose:
-            print("built", out)
-        outs.append(out)
-    return outs
-
-
 if __name__ == "__main__":
     build(verbose=True)
-    if "--probe" in sys.argv[1:]:
-        build_probe(verbose=True)
     sys.exit(0)

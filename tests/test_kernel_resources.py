@@ -2,7 +2,7 @@
 remarks (hipcc -Rpass-analysis=kernel-resource-usage on the device pass).
 
 A CU of gfx950 holds four 512-thread workgroups of the train kernel only at
-64 VGPRs or fewer and below about 80 SGPRs (measured with probe_residency:
+64 VGPRs or fewer and below about 80 SGPRs (a recorded residency measurement:
 four fit at 70 SGPRs, three at 86); three at more.  The workload-2 tile
 (train_kernel_r4<4, step queue>) is built to that budget, and the encoder
 tile (64 lanes x 8 words) to 128 VGPRs for two workgroups per CU; neither
