@@ -203,6 +203,23 @@ int tm_checkpoint(tm_machine *m, uint8_t *buf, size_t cap, size_t *len);
  * unless len == action_chunks. */
 int tm_restore(tm_machine *m, const uint8_t *buf, size_t len);
 
+/* checkpoint/restore with the buffer in device memory (same byte layout as
+ * tm_checkpoint/tm_restore, capi.c:1564-1588).  No reference counterpart;
+ * for callers that keep a best-epoch checkpoint in HBM (a search trial's
+ * early stop, optimize.lua:616,648) without a round trip through the host. */
+int tm_checkpoint_dev(tm_machine *m, uint8_t *buf_dev, size_t cap, size_t *len);
+int tm_restore_dev(tm_machine *m, const uint8_t *buf_dev, size_t len);
+
+/* The counters in the reference's layout: state_chunks bytes,
+ * counts[clause][plane][input_chunks] with plane 0 the least significant bit
+ * (automata.h:7-31; an individualized machine's classes one after another,
+ * capi.c:576-593).  No reference counterpart: the reference neither persists
+ * nor returns its counts (capi.c:1509-1531); this is a diagnostic read for
+ * parity checks of the whole automata state.  TM_E_STATE when the machine
+ * holds no counters (loaded, or released after training); TM_E_SIZE if cap
+ * is too small (*len always receives the size). */
+int tm_counts(tm_machine *m, uint8_t *buf, size_t cap, size_t *len);
+
 /* machine:reconfigure(opts) -> tk_tsetlin_reconfigure, capi.c:1590-1673.
  * Reads clauses, clause_tolerance, clause_maximum, include_bits, target,
  * specificity (and seed) from opts; TM_E_STATE unless created reusable. */
@@ -312,6 +329,12 @@ int tm_batch_train_classifier_dev(tm_batch *b, uint32_t n, const uint8_t *proble
 /* Held-out accuracy of every trial on device-resident rows and labels. */
 int tm_batch_accuracy_dev(tm_batch *b, const uint8_t *problems_dev, const int64_t *solutions_dev, uint32_t n,
                           double *accuracy);
+/* evaluator.class_accuracy of every trial (evaluator/capi.c:119-203, the
+ * search metric of the reference's classifier specs, classify/mnist.lua:93-97):
+ * macro[3 t .. 3 t + 2] = macro precision, recall, F1 of trial t, the
+ * reference's doubles.  One synchronisation for all trials. */
+int tm_batch_class_accuracy_dev(tm_batch *b, const uint8_t *problems_dev, const int64_t *solutions_dev, uint32_t n,
+                                double *macro);
 /* The autotuner's choice: lanes per clause, words per lane, workgroup order
  * (0 trial-major, 1 heaviest trials first) and its epoch time. */
 int tm_batch_tuning(const tm_batch *b, uint32_t *lanes, uint32_t *wpl, uint32_t *order, double *epoch_ms);

@@ -171,16 +171,37 @@ int use_device(tm_machine *m)
   return TM_OK;
 }
 
+// Widths of the kernels' integer arithmetic.  A vote is at most the
+// tolerance and a predict tile sums 64 of them in 32 bits; the train kernel's
+// fire test computes T +- v and 2T in 32 bits.  The reference computes in
+// long int and double (capi.c:148-161, :217-225) and has no such limits; class
+// sums here are 64-bit as there, so only these two remain.
+const uint32_t kMaxTolerance = (1u << 25) - 1;
+const double kMaxTarget = (double) ((1u << 30) - 1);
+
 // capi.c:473-493: derived sizes and defaults.  The target keeps the reference's
 // float fmaxf and truncating conversion to unsigned int (capi.c:480-483).
-uint32_t derive_target(double targetf, uint32_t clauses, uint32_t tolerance)
+// Targets above kMaxTarget are refused before the conversion.
+int derive_target(double targetf, uint32_t clauses, uint32_t tolerance, const char *what, uint32_t *out)
 {
   double t;
   if (targetf < 0)
     t = sqrt((double) clauses / 2.0) * (double) tolerance;
   else
     t = ceil(targetf >= 1 ? targetf : fmaxf(1.0f, (float)((double) clauses * targetf)));
-  return (uint32_t) t;
+  if (!(t <= kMaxTarget))
+    return fail(TM_E_UNSUPPORTED, "%s: target %.17g exceeds %.0f, the train kernel's 32-bit feedback arithmetic",
+                what, t, kMaxTarget);
+  *out = (uint32_t) t;
+  return TM_OK;
+}
+
+int check_tolerance(uint32_t tolerance, const char *what)
+{
+  if (tolerance > kMaxTolerance)
+    return fail(TM_E_UNSUPPORTED, "%s: clause_tolerance %u exceeds %u, the kernels' 32-bit vote sums", what,
+                tolerance, kMaxTolerance);
+  return TM_OK;
 }
 
 // capi.c:463-472 (plain) and :542-553 (individualized: no features option;
@@ -199,7 +220,7 @@ int validate_opts(const tm_options *o, const char *what, bool ind = false)
     return fail(TM_E_ARG, "%s: individualized machines are created by the _ind entry points (feat_offsets)", what);
   if (o->state > TM_MAX_STATE_BITS)
     return fail(TM_E_UNSUPPORTED, "%s: state bits above %d are not built in this engine", what, TM_MAX_STATE_BITS);
-  return TM_OK;
+  return check_tolerance(o->clause_tolerance, what);
 }
 
 // Size limits of a machine; `train` adds the train tile's (a loaded machine
@@ -210,8 +231,6 @@ int size_check(uint64_t classes, uint64_t clauses, uint64_t input_chunks, uint64
   uint64_t sta = act * (state_bits - 1);
   if (act > 0xFFFFFFFFull || sta > 0xFFFFFFFFull)
     return fail(TM_E_ARG, "machine too large for the persisted 32-bit size fields");
-  if ((uint64_t) clauses * 64ull > 0x7FFFFFFFull)
-    return fail(TM_E_ARG, "clauses per class too large for 32-bit class sums");
   uint32_t lanes, wpl;
   if (train && tm_train_shape((uint32_t)((input_chunks + 3) / 4), (uint32_t) state_bits, &lanes, &wpl))
     return fail(TM_E_UNSUPPORTED, "rows of %llu bytes exceed the largest train tile for %llu state bits (%u bytes)",
@@ -301,7 +320,7 @@ int create_common(const tm_options *o, int type, tm_machine **out, hipStream_t s
   m->clauses = ((o->clauses + 7) / 8) * 8;
   m->clause_tolerance = o->clause_tolerance;
   m->clause_maximum = o->clause_maximum;
-  m->target = derive_target(o->target, m->clauses, o->clause_tolerance);
+  if ((rc = derive_target(o->target, m->clauses, o->clause_tolerance, what, &m->target))) { delete m; return rc; }
   m->features = o->features;
   m->state_bits = o->state;
   m->include_bits = o->include_bits ? o->include_bits : 1;
@@ -348,7 +367,7 @@ int create_ind(const tm_options *o, const int64_t *feat_offsets, uint32_t n_offs
   m->clause_chunks = m->clauses / 8;
   m->clause_tolerance = o->clause_tolerance;
   m->clause_maximum = o->clause_maximum;
-  m->target = derive_target(o->target, m->clauses, o->clause_tolerance);
+  if ((rc = derive_target(o->target, m->clauses, o->clause_tolerance, what, &m->target))) { delete m; return rc; }
   m->state_bits = o->state;
   m->include_bits = o->include_bits ? o->include_bits : 1;
   m->specificity = o->specificity;
@@ -489,8 +508,8 @@ int predict_sums_ind(tm_machine *m, const uint8_t *rows, size_t bytes, const int
   for (uint32_t h = 0; h < m->classes; h++) strides[h] = m->ind[h].wd;
   std::vector<uint64_t> woff;
   if ((rc = stage_rows_ind(m, rows, n, extent, dim_offsets, strides, m->p_rows, m->p_words, woff))) return rc;
-  if ((rc = m->p_sums.ensure((size_t) n * m->classes * 4))) return rc;
-  HIP_TRY(tm_launch_zero(m->p_sums.p, (size_t) n * m->classes * 4, m->stream));
+  if ((rc = m->p_sums.ensure((size_t) n * m->classes * 8))) return rc;
+  HIP_TRY(tm_launch_zero(m->p_sums.p, (size_t) n * m->classes * 8, m->stream));
   HIP_TRY(hipEventRecord(m->ev0, m->stream));
   for (uint32_t h = 0; h < m->classes; h++) {
     const tm_ind_class &c = m->ind[h];
@@ -499,7 +518,7 @@ int predict_sums_ind(tm_machine *m, const uint8_t *rows, size_t bytes, const int
     L.actT = m->actT.as<uint32_t>() + c.aoff;
     L.lit = m->lit.as<uint32_t>() + (size_t) h * m->clauses;
     L.rows = m->p_words.as<uint32_t>() + woff[h];
-    L.sums = m->p_sums.as<int32_t>();
+    L.sums = m->p_sums.as<int64_t>();
     L.n = n;
     L.wd = c.wd;
     L.clauses_total = m->clauses;
@@ -522,14 +541,14 @@ int predict_sums(tm_machine *m, const uint8_t *rows, uint32_t n, bool dev)
   if ((rc = prepare_predict(m))) return rc;
   const uint32_t *words;
   if ((rc = stage_rows(m, rows, n, dev, m->wd, m->p_rows, m->p_words, &words))) return rc;
-  if ((rc = m->p_sums.ensure((size_t) n * m->classes * 4))) return rc;
-  HIP_TRY(tm_launch_zero(m->p_sums.p, (size_t) n * m->classes * 4, m->stream));
+  if ((rc = m->p_sums.ensure((size_t) n * m->classes * 8))) return rc;
+  HIP_TRY(tm_launch_zero(m->p_sums.p, (size_t) n * m->classes * 8, m->stream));
   tm_predict_launch L;
   memset(&L, 0, sizeof L);
   L.actT = m->actT.as<uint32_t>();
   L.lit = m->lit.as<uint32_t>();
   L.rows = words;
-  L.sums = m->p_sums.as<int32_t>();
+  L.sums = m->p_sums.as<int64_t>();
   L.n = n;
   L.wd = m->wd;
   L.clauses_total = (uint32_t) m->nclauses();
@@ -557,7 +576,7 @@ int predict_enqueue_dev(tm_machine *m, const uint8_t *rows, uint32_t n, int64_t 
 {
   int rc;
   if ((rc = predict_sums(m, rows, n, true))) return rc;
-  HIP_TRY(tm_launch_argmax(m->p_sums.as<int32_t>(), n, m->classes, out_dev, m->stream));
+  HIP_TRY(tm_launch_argmax(m->p_sums.as<int64_t>(), n, m->classes, out_dev, m->stream));
   return TM_OK;
 }
 
@@ -582,9 +601,9 @@ int predict_impl(tm_machine *m, const uint8_t *rows, uint32_t n, void *out, bool
     dout = m->p_out.p;
   }
   if (want_type == TM_CLASSIFIER)
-    HIP_TRY(tm_launch_argmax(m->p_sums.as<int32_t>(), n, m->classes, (int64_t *) dout, m->stream));
+    HIP_TRY(tm_launch_argmax(m->p_sums.as<int64_t>(), n, m->classes, (int64_t *) dout, m->stream));
   else
-    HIP_TRY(tm_launch_encode(m->p_sums.as<int32_t>(), n, m->classes, (uint8_t *) dout, m->stream));
+    HIP_TRY(tm_launch_encode(m->p_sums.as<int64_t>(), n, m->classes, (uint8_t *) dout, m->stream));
   if (!dev) HIP_TRY(hipMemcpyAsync(out, dout, out_bytes, hipMemcpyDeviceToHost, m->stream));
   return predict_finish_timing(m);
 }
@@ -903,37 +922,45 @@ std::vector<action_run> action_runs(const tm_machine *m)
   return r;
 }
 
-int actions_to_host(tm_machine *m, uint8_t *dst)
+// Copy the include plane (planes = 1) or the count planes (planes = M)
+// between the device layout, [clause][plane][wd] u32 words per run, and the
+// reference's, [clause][plane][input_chunks] bytes per run (automata.h:7-31;
+// an individualized machine's classes one after another, capi.c:576-593).
+// `ref` is host or device memory; to_ref picks the direction.  Device bytes
+// past a row's input_chunks are not part of the reference layout: a copy into
+// the device leaves them zero.
+int layout_copy(tm_machine *m, uint8_t *ref, bool ref_dev, bool to_ref, bool counts)
 {
-  // device [clause][wd] words -> reference layout [clause][input_chunks] bytes
+  const uint64_t planes = counts ? m->state_bits - 1 : 1;
+  const dbuf &dev = counts ? m->counts : m->actions;
   const std::vector<action_run> runs = action_runs(m);
-  if (runs.size() == 1 && runs[0].bytes == runs[0].wd * 4) {
-    HIP_TRY(hipMemcpyAsync(dst, m->actions.p, runs[0].clauses * runs[0].bytes, hipMemcpyDeviceToHost, m->stream));
-    HIP_TRY(hipStreamSynchronize(m->stream));
-    return TM_OK;
+  const hipMemcpyKind kind = to_ref ? (ref_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost)
+                                    : (ref_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice);
+  bool padded = false;
+  for (const action_run &r : runs) padded |= r.bytes != r.wd * 4;
+  if (!to_ref && padded) HIP_TRY(tm_launch_zero(dev.p, m->action_words() * planes * 4, m->stream));
+  for (const action_run &r : runs) {
+    uint8_t *d = dev.as<uint8_t>() + r.aoff * planes * 4;
+    uint8_t *h = ref + r.hoff * planes;
+    const size_t height = r.clauses * planes, pitch = r.wd * 4;
+    if (r.bytes == pitch)
+      HIP_TRY(hipMemcpyAsync(to_ref ? h : d, to_ref ? d : h, height * pitch, kind, m->stream));
+    else if (to_ref)
+      HIP_TRY(hipMemcpy2DAsync(h, r.bytes, d, pitch, r.bytes, height, kind, m->stream));
+    else
+      HIP_TRY(hipMemcpy2DAsync(d, pitch, h, r.bytes, r.bytes, height, kind, m->stream));
   }
-  std::vector<uint32_t> tmp(m->action_words());
-  HIP_TRY(hipMemcpyAsync(tmp.data(), m->actions.p, tmp.size() * 4, hipMemcpyDeviceToHost, m->stream));
   HIP_TRY(hipStreamSynchronize(m->stream));
-  for (const action_run &r : runs)
-    for (uint64_t c = 0; c < r.clauses; c++) memcpy(dst + r.hoff + c * r.bytes, &tmp[r.aoff + c * r.wd], r.bytes);
   return TM_OK;
 }
 
+int actions_to_host(tm_machine *m, uint8_t *dst) { return layout_copy(m, dst, false, true, false); }
+
 int actions_from_host(tm_machine *m, const uint8_t *src)
 {
-  const std::vector<action_run> runs = action_runs(m);
-  if (runs.size() == 1 && runs[0].bytes == runs[0].wd * 4) {
-    HIP_TRY(hipMemcpyAsync(m->actions.p, src, runs[0].clauses * runs[0].bytes, hipMemcpyHostToDevice, m->stream));
-  } else {
-    std::vector<uint32_t> tmp(m->action_words(), 0);
-    for (const action_run &r : runs)
-      for (uint64_t c = 0; c < r.clauses; c++) memcpy(&tmp[r.aoff + c * r.wd], src + r.hoff + c * r.bytes, r.bytes);
-    HIP_TRY(hipMemcpyAsync(m->actions.p, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice, m->stream));
-  }
-  HIP_TRY(hipStreamSynchronize(m->stream));
+  int rc = layout_copy(m, const_cast<uint8_t *>(src), false, false, false);
   m->predict_ready = false;
-  return TM_OK;
+  return rc;
 }
 
 // capi.c:1509-1531 (+ the int32 type written first at capi.c:1543).
@@ -1211,10 +1238,11 @@ int tm_load(const uint8_t *data, size_t len, tm_machine **out)
     delete m;
     return fail(TM_E_FORMAT, "load: inconsistent header");
   }
+  int rc;
+  if ((rc = check_tolerance(m->clause_tolerance, "load"))) { delete m; return rc; }
   std::vector<uint8_t> acts(m->action_chunks);
   uint8_t ind = 0;
   if (!r.take(acts.data(), acts.size()) || !r.take(&ind, 1)) { delete m; return fail(TM_E_FORMAT, "load: truncated actions"); }
-  int rc;
   if (ind) {
     // capi.c:1698-1725: the feature sizes follow; the per-class layout is
     // derived from them
@@ -1286,6 +1314,45 @@ int tm_restore(tm_machine *m, const uint8_t *buf, size_t len)
   return actions_from_host(m, buf);
 }
 
+int tm_checkpoint_dev(tm_machine *m, uint8_t *buf_dev, size_t cap, size_t *len)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (!len) return fail(TM_E_ARG, "checkpoint: len is NULL");
+  *len = m->action_chunks;
+  if (!buf_dev || cap < m->action_chunks)
+    return fail(TM_E_SIZE, "checkpoint: buffer of %zu bytes, need %u", cap, m->action_chunks);
+  if ((rc = use_device(m))) return rc;
+  return layout_copy(m, buf_dev, true, true, false);
+}
+
+int tm_restore_dev(tm_machine *m, const uint8_t *buf_dev, size_t len)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (len != m->action_chunks)
+    return fail(TM_E_SIZE, "checkpoint size mismatch: expected %u bytes, got %zu", m->action_chunks, len);
+  if (!buf_dev) return fail(TM_E_ARG, "restore: buffer is NULL");
+  if ((rc = use_device(m))) return rc;
+  rc = layout_copy(m, const_cast<uint8_t *>(buf_dev), true, false, false);
+  m->predict_ready = false;
+  return rc;
+}
+
+int tm_counts(tm_machine *m, uint8_t *buf, size_t cap, size_t *len)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (!len) return fail(TM_E_ARG, "counts: len is NULL");
+  *len = m->state_chunks;
+  if (!m->has_state)
+    return fail(TM_E_STATE, "counts: the machine holds no counters (loaded, or released after training; create "
+                            "with reusable=true to keep them)");
+  if (!buf || cap < m->state_chunks) return fail(TM_E_SIZE, "counts: buffer of %zu bytes, need %u", cap, m->state_chunks);
+  if ((rc = use_device(m))) return rc;
+  return layout_copy(m, buf, false, true, true);
+}
+
 // capi.c:1590-1673.
 int tm_reconfigure(tm_machine *m, const tm_options *o)
 {
@@ -1297,8 +1364,11 @@ int tm_reconfigure(tm_machine *m, const tm_options *o)
   if (!o->clause_tolerance) return fail(TM_E_ARG, "reconfigure: clause_tolerance must be greater than 0");
   if (!o->clause_maximum) return fail(TM_E_ARG, "reconfigure: clause_maximum must be greater than 0");
   if (!(o->specificity > 0)) return fail(TM_E_ARG, "reconfigure: specificity must be a positive number");
+  if ((rc = check_tolerance(o->clause_tolerance, "reconfigure"))) return rc;
   if ((rc = use_device(m))) return rc;
   const uint32_t nclauses = ((o->clauses + 7) / 8) * 8;
+  uint32_t target;
+  if ((rc = derive_target(o->target, nclauses, o->clause_tolerance, "reconfigure", &target))) return rc;
   if (m->individualized) {
     // capi.c:1609-1615, :1651-1663: the per-class layout follows the new
     // clause count
@@ -1320,7 +1390,7 @@ int tm_reconfigure(tm_machine *m, const tm_options *o)
   m->clause_maximum = o->clause_maximum;
   m->include_bits = o->include_bits ? o->include_bits : 1;
   m->specificity = o->specificity;
-  m->target = derive_target(o->target, nclauses, o->clause_tolerance);
+  m->target = target;
   m->seed = o->seed;
   if ((rc = alloc_device(m))) return rc;   // actions zeroed, counts (re)allocated
   m->predict_ready = false;
@@ -1408,8 +1478,12 @@ struct tm_batch {
 
 namespace {
 
+// Autotuner choices keyed by (row words, state bits, row stride, trials,
+// workgroups): the candidate tiles depend on the first two, and a tile must
+// divide the stride the rows were staged with.
 std::mutex g_tune_mu;
-std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::tuple<uint32_t, uint32_t, uint32_t, double>> g_tune_cache;
+std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, std::tuple<uint32_t, uint32_t, uint32_t, double>>
+    g_tune_cache;
 
 void batch_release_graph(tm_batch *b)
 {
@@ -1559,14 +1633,21 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   // autotune on epoch 0 (every candidate computes the same epoch-0 state)
   uint64_t nblocks = 0;
   for (uint32_t t = 0; t < nt; t++) nblocks += (uint64_t) T[t].clause_chunks * b->m[t]->classes;
-  const auto key = std::make_tuple(m0->wd, nt, (uint32_t) nblocks);
+  const auto key = std::make_tuple(m0->wd, m0->state_bits, stride, nt, (uint32_t) nblocks);
   bool cached = false;
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
-      std::tie(b->lanes, b->wpl, b->order, b->tuned_ms) = it->second;
-      cached = true;
+      uint32_t lanes, wpl, order;
+      double ms;
+      std::tie(lanes, wpl, order, ms) = it->second;
+      // only a tile among this batch's candidates is reused
+      for (int i = 0; i < nshapes; i++)
+        if (shapes[i][0] == lanes && shapes[i][1] == wpl && stride % wpl == 0) {
+          b->lanes = lanes; b->wpl = wpl; b->order = order; b->tuned_ms = ms;
+          cached = true;
+        }
     }
   }
   if (!cached) {
@@ -1722,6 +1803,39 @@ int tm_batch_accuracy_dev(tm_batch *b, const uint8_t *problems, const int64_t *s
   HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->eval_ms = ms;
   for (uint32_t t = 0; t < nt; t++) accuracy[t] = (double) h[t] / (double) n;
+  return TM_OK;
+}
+
+int tm_batch_class_accuracy_dev(tm_batch *b, const uint8_t *problems, const int64_t *solutions, uint32_t n,
+                                double *macro)
+{
+  if (!b || !macro) return fail(TM_E_ARG, "batch class accuracy: NULL argument");
+  const uint32_t nt = (uint32_t) b->m.size(), K = b->m[0]->classes;
+  if (n == 0) {
+    for (size_t i = 0; i < (size_t) 3 * nt; i++) macro[i] = 0;
+    return TM_OK;
+  }
+  if (!problems || !solutions) return fail(TM_E_ARG, "batch class accuracy: problems and solutions must not be NULL");
+  HIP_TRY(hipSetDevice(b->device));
+  int rc;
+  const size_t per = (size_t) 3 * K;
+  if ((rc = b->hits.ensure((size_t) nt * per * 8))) return rc;
+  if ((rc = b->pred.ensure((size_t) n * 8))) return rc;
+  HIP_TRY(tm_launch_zero(b->hits.p, (size_t) nt * per * 8, b->stream));
+  HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  for (uint32_t t = 0; t < nt; t++) {
+    if ((rc = predict_enqueue_dev(b->m[t], problems, n, b->pred.as<int64_t>()))) return rc;
+    HIP_TRY(tm_launch_class_counts(b->pred.as<int64_t>(), solutions, n, K,
+                                   b->hits.as<unsigned long long>() + t * per, b->stream));
+  }
+  HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  std::vector<uint64_t> h((size_t) nt * per);
+  HIP_TRY(hipMemcpyAsync(h.data(), b->hits.p, h.size() * 8, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->eval_ms = ms;
+  for (uint32_t t = 0; t < nt; t++) tm_finish_class_accuracy(&h[(size_t) t * per], K, nullptr, macro + 3 * t);
   return TM_OK;
 }
 

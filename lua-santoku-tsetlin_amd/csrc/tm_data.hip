@@ -187,6 +187,33 @@ __global__ void bit_diff_kernel(const uint8_t *pred, const uint8_t *expect, uint
 
 } // namespace
 
+// evaluator/capi.c:156-173, in its order: counts [tp][fp][fn] per class.
+void tm_finish_class_accuracy(const uint64_t *h, uint32_t classes, double *per_class, double *macro)
+{
+  double pa = 0.0, ra = 0.0, fa = 0.0;
+  for (uint32_t c = 0; c < classes; c++) {
+    const uint64_t t = h[c], f = h[classes + c], m = h[2 * (size_t) classes + c];
+    const double p = (t + f) > 0 ? (double) t / (t + f) : 0.0;
+    const double r = (t + m) > 0 ? (double) t / (t + m) : 0.0;
+    const double f1 = (p + r) > 0 ? 2.0 * p * r / (p + r) : 0.0;
+    if (per_class) { per_class[3 * c] = p; per_class[3 * c + 1] = r; per_class[3 * c + 2] = f1; }
+    pa += p;
+    ra += r;
+    fa += f1;
+  }
+  if (macro) { macro[0] = pa / classes; macro[1] = ra / classes; macro[2] = fa / classes; }
+}
+
+int tm_launch_class_counts(const int64_t *pred, const int64_t *expect, uint32_t n, uint32_t classes,
+                           unsigned long long *counts, hipStream_t st)
+{
+  if (n == 0) return (int) hipSuccess;
+  unsigned long long *tp = counts, *fp = tp + classes, *fn = fp + classes;
+  hipLaunchKernelGGL(class_counts_kernel, dim3(grid_for(n, 256) < 1024 ? grid_for(n, 256) : 1024), dim3(256), 0, st,
+                     pred, expect, (uint64_t) n, classes, tp, fp, fn);
+  return (int) hipGetLastError();
+}
+
 extern "C" {
 
 int tm_bits_to_cvec_dev(const int64_t *set_bits_dev, uint64_t n_set, uint32_t n, uint32_t features, uint8_t *rows_dev,
@@ -254,19 +281,7 @@ int tm_class_accuracy_dev(const int64_t *predicted_dev, const int64_t *expected_
   std::vector<uint64_t> h((size_t) 3 * classes);
   HIP_TRY(hipMemcpyAsync(h.data(), cnt.p, h.size() * 8, hipMemcpyDeviceToHost, 0));
   HIP_TRY(hipStreamSynchronize(0));
-  // evaluator/capi.c:156-173, in its order
-  double pa = 0.0, ra = 0.0, fa = 0.0;
-  for (uint32_t c = 0; c < classes; c++) {
-    const uint64_t t = h[c], f = h[classes + c], m = h[2 * (size_t) classes + c];
-    const double p = (t + f) > 0 ? (double) t / (t + f) : 0.0;
-    const double r = (t + m) > 0 ? (double) t / (t + m) : 0.0;
-    const double f1 = (p + r) > 0 ? 2.0 * p * r / (p + r) : 0.0;
-    if (per_class) { per_class[3 * c] = p; per_class[3 * c + 1] = r; per_class[3 * c + 2] = f1; }
-    pa += p;
-    ra += r;
-    fa += f1;
-  }
-  if (macro) { macro[0] = pa / classes; macro[1] = ra / classes; macro[2] = fa / classes; }
+  tm_finish_class_accuracy(h.data(), classes, per_class, macro);
   return TM_OK;
 }
 

@@ -87,11 +87,11 @@ struct tm_predict_launch {
   const uint32_t *actT;    // [Wd][clauses_total] transposed include plane
   const uint32_t *lit;     // [clauses_total] literal counts
   const uint32_t *rows;    // [n][Wd]
-  int32_t *sums;           // [n][sums_stride]; class k of this launch adds into column class_offset + k
+  int64_t *sums;           // [n][sums_stride]; class k of this launch adds into column class_offset + k
   uint32_t n, wd, clauses_total, clauses, classes, tolerance, last_eval;
   uint32_t sums_stride;    // 0: classes
   uint32_t class_offset;
-  uint32_t pass_tiles;     // set by the launcher: 64-clause tiles per L2-resident pass
+  uint32_t pass_groups;    // set by the launcher: tile groups per L2-resident pass
 };
 
 // Record `msg` as the calling thread's tm_last_error() text; returns code.
@@ -112,8 +112,8 @@ int tm_launch_queue_write(const tm_queue_launch &Q, hipStream_t st);
 int tm_launch_prepare_predict(const uint32_t *actions, uint32_t *actT, uint32_t *lit, uint32_t clauses_total,
                               uint32_t wd, uint32_t last_eval, hipStream_t st);
 int tm_launch_predict(const tm_predict_launch &L, hipStream_t st);
-int tm_launch_argmax(const int32_t *sums, uint32_t n, uint32_t classes, int64_t *out, hipStream_t st);
-int tm_launch_encode(const int32_t *sums, uint32_t n, uint32_t classes, uint8_t *out, hipStream_t st);
+int tm_launch_argmax(const int64_t *sums, uint32_t n, uint32_t classes, int64_t *out, hipStream_t st);
+int tm_launch_encode(const int64_t *sums, uint32_t n, uint32_t classes, uint8_t *out, hipStream_t st);
 int tm_launch_repack(const uint8_t *src, uint32_t n, uint32_t bytes, uint32_t stride_words, uint32_t *dst, hipStream_t st);
 int tm_launch_zero(void *p, size_t bytes, hipStream_t st);
 // State bits the train kernels are built for: 2 .. TM_MAX_STATE_BITS.  A tile
@@ -128,3 +128,10 @@ int tm_launch_epoch_advance(uint32_t *epoch_dev, hipStream_t st);
 // hits[t] += number of s < n with pred[s] == labels[s] (one counter per launch)
 int tm_launch_count_hits(const int64_t *pred, const int64_t *labels, uint32_t n, unsigned long long *hits,
                          hipStream_t st);
+// Per-class TP / FP / FN of predictions against labels (classes outside
+// [0, classes) skipped, the reference's unsigned conversion) added into
+// counts[3][classes] on stream st; and the host finish of
+// evaluator/capi.c:156-173 over such counts, in the reference's order.
+int tm_launch_class_counts(const int64_t *pred, const int64_t *expect, uint32_t n, uint32_t classes,
+                           unsigned long long *counts, hipStream_t st);
+void tm_finish_class_accuracy(const uint64_t *counts, uint32_t classes, double *per_class, double *macro);

@@ -72,127 +72,170 @@ __global__ void literal_count_kernel(const uint32_t *actions, uint32_t *lit, uin
 constexpr int PRED_WAVES = 4;    // waves per workgroup
 constexpr int PRED_SPW = 16;     // samples per wave
 constexpr int PRED_WC = 32;      // words per staged slice
+constexpr int PRED_TPG = 8;      // 64-clause tiles per workgroup (a tile group)
 
+// A workgroup evaluates one tile group (up to PRED_TPG consecutive 64-clause
+// tiles) against 64 samples.  Each tile's votes are summed per class over its
+// 64 lanes (a DPP reduction to a scalar), and the sums of consecutive tiles of
+// one class are kept per sample in scalar registers, so a (sample, class) gets
+// one 64-bit atomic add per tile group rather than one per tile.
 __global__ __launch_bounds__(64 * PRED_WAVES) void predict_kernel(tm_predict_launch L)
 {
   __shared__ uint4 xs[PRED_WAVES * PRED_SPW][PRED_WC / 4];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // logical tile order: passes of `pass_tiles` clause tiles; within a pass every
-  // sample tile, and within that the pass's clause tiles (their include plane
-  // stays in the XCD's L2 while the rows stream past)
+  // logical order: passes of `pass_groups` tile groups (about 2 MiB of the
+  // include plane); within a pass every sample tile, and within that the
+  // pass's groups (their include plane stays in the XCD's L2 while the rows
+  // stream past)
   const uint32_t nst = (L.n + PRED_WAVES * PRED_SPW - 1) / (PRED_WAVES * PRED_SPW);
   const uint32_t nct = (L.clauses_total + 63) / 64;
-  const uint32_t lb = xcd_logical(blockIdx.x, nst * nct);
-  const uint32_t tp = L.pass_tiles, full = nct / tp, in_full = full * tp * nst;
-  uint32_t st_i, ct_i;
+  const uint32_t ngr = (nct + PRED_TPG - 1) / PRED_TPG;
+  const uint32_t lb = xcd_logical(blockIdx.x, nst * ngr);
+  const uint32_t gp = L.pass_groups, full = ngr / gp, in_full = full * gp * nst;
+  uint32_t st_i, gr_i;
   if (lb < in_full) {
-    const uint32_t pass = lb / (tp * nst), r = lb % (tp * nst);
-    st_i = r / tp;
-    ct_i = pass * tp + r % tp;
+    const uint32_t pass = lb / (gp * nst), r = lb % (gp * nst);
+    st_i = r / gp;
+    gr_i = pass * gp + r % gp;
   } else {
-    const uint32_t r = lb - in_full, w = nct - full * tp;
+    const uint32_t r = lb - in_full, w = ngr - full * gp;
     st_i = r / w;
-    ct_i = full * tp + r % w;
+    gr_i = full * gp + r % w;
   }
-  const uint32_t c = ct_i * 64 + lane;
+  const uint32_t ct0 = gr_i * PRED_TPG, ct1 = min(nct, ct0 + PRED_TPG);
   const uint32_t s0 = st_i * (PRED_WAVES * PRED_SPW);
-  const bool cvalid = c < L.clauses_total;
-  uint32_t acc[PRED_SPW];
-#pragma unroll
-  for (int i = 0; i < PRED_SPW; i++) acc[i] = 0;
 
-  // each thread stages STG words of a slice; the next slice's words are
-  // loaded while this one is evaluated (all-ones stands for a word past the
-  // rows, so its complement is zero)
-  // Thread tid stages column tid % PRED_WC of rows s0 + tid / PRED_WC + k * RS.
+  // each thread stages STG words of a slice; the next slice's words (or the
+  // next tile's first slice: the rows are the same for every tile) are loaded
+  // while this one is evaluated.  Thread tid stages column tid % PRED_WC of
+  // rows s0 + tid / PRED_WC + k * RS.  Rows and the include plane are read
+  // with buffer loads: a descriptor with a scalar base, a 32-bit lane offset
+  // and a scalar offset, where 64-bit per-lane addresses would take two
+  // VGPRs per load in flight.  Every offset is clamped into its buffer: a row
+  // past the tile's last sample re-reads the last one (its sums are never
+  // written), and a word past the row's end is masked out when staged.
   constexpr uint32_t STG = PRED_SPW * PRED_WC / 64, RS = 64 * PRED_WAVES / PRED_WC;
   static_assert(PRED_WAVES * PRED_SPW * PRED_WC == STG * 64 * PRED_WAVES && (64 * PRED_WAVES) % PRED_WC == 0,
                 "slice staging");
-  const uint32_t srow = s0 + tid / PRED_WC, wcol = tid % PRED_WC;
-  const uint32_t *const rb = L.rows + (size_t) min(srow, L.n - 1) * L.wd;
+  const uint32_t wcol = tid % PRED_WC;
+  const uint32_t rv = min(L.n - s0, (uint32_t)(PRED_WAVES * PRED_SPW));   // rows of this sample tile
+  const __amdgpu_buffer_rsrc_t rrs = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(L.rows + (size_t) s0 * L.wd), (short) 0, (int)(rv * L.wd * 4), 0x00020000);
   uint32_t pre[STG];
   auto fetch_slice = [&](uint32_t w0) {
-    const uint32_t w = w0 + wcol;
+    // the row offsets are recomputed per slice (an empty asm keeps the
+    // compiler from holding all of them across the loops)
+    uint32_t r0 = tid / PRED_WC;
+    asm volatile("" : "+v"(r0));
+    const uint32_t wc = min(w0 + wcol, L.wd - 1) * 4;
 #pragma unroll
     for (uint32_t k = 0; k < STG; k++)
-      pre[k] = (srow + k * RS < L.n && w < L.wd) ? rb[k * RS * L.wd + w] : 0xFFFFFFFFu;
+      pre[k] = __builtin_amdgcn_raw_buffer_load_b32(rrs, min(r0 + k * RS, rv - 1) * (L.wd * 4) + wc, 0, 0);
   };
   fetch_slice(0);
-  for (uint32_t w0 = 0; w0 < L.wd; w0 += PRED_WC) {
-    __syncthreads();
-    uint32_t *xw = reinterpret_cast<uint32_t *>(&xs[0][0]);
-    const uint32_t wm = word_mask(w0 + wcol, L.wd, L.last_eval);
-#pragma unroll
-    for (uint32_t k = 0; k < STG; k++) xw[tid + k * 64 * PRED_WAVES] = ~pre[k] & wm;
-    __syncthreads();
-    if (w0 + PRED_WC < L.wd) fetch_slice(w0 + PRED_WC);
-    uint32_t a[PRED_WC];
-#pragma unroll
-    for (int ww = 0; ww < PRED_WC; ww++) {
-      const uint32_t w = w0 + ww;
-      a[ww] = (cvalid && w < L.wd) ? L.actT[(size_t) w * L.clauses_total + c] : 0u;
-    }
-#pragma unroll
-    for (int si = 0; si < PRED_SPW; si++) {
-      const uint32_t sl = wave * PRED_SPW + si;
-      // two accumulators keep two independent popcount chains in flight
-      uint32_t t0 = acc[si], t1 = 0;
-#pragma unroll
-      for (int q = 0; q < PRED_WC / 4; q++) {
-        const uint4 xv = xs[sl][q];
-        t0 = bcnt_acc(a[4 * q + 0] & xv.x, t0);
-        t1 = bcnt_acc(a[4 * q + 1] & xv.y, t1);
-        t0 = bcnt_acc(a[4 * q + 2] & xv.z, t0);
-        t1 = bcnt_acc(a[4 * q + 3] & xv.w, t1);
-      }
-      acc[si] = t0 + t1;
-    }
-  }
 
-  // votes (capi.c:110-119) and class sums (capi.c:928-933)
-  const uint32_t lit = cvalid ? L.lit[c] : 0u;
-  const uint32_t cbase = ct_i * 64;
-  const uint32_t cls_first = cbase / L.clauses;
-  uint32_t clast = cbase + 63;
-  if (clast >= L.clauses_total) clast = L.clauses_total - 1;
-  const uint32_t cls_last = clast / L.clauses;
-  const uint32_t my_cls = c / L.clauses;
+  // running sums for class rc (wave-uniform) of this wave's samples: lane si
+  // holds sample si's
+  int64_t run = 0;
+  uint32_t rc = 0xFFFFFFFFu;
+  auto flush = [&]() {
+    const uint32_t s = s0 + wave * PRED_SPW + lane;
+    if (rc != 0xFFFFFFFFu && lane < PRED_SPW && s < L.n)
+      atomicAdd(reinterpret_cast<unsigned long long *>(&L.sums[(size_t) s * L.sums_stride + L.class_offset + rc]),
+                (unsigned long long) run);
+  };
+
+  for (uint32_t ct = ct0; ct < ct1; ct++) {
+    const uint32_t c = ct * 64 + lane;
+    const bool cvalid = c < L.clauses_total;
+    uint32_t acc[PRED_SPW];
 #pragma unroll
-  for (int si = 0; si < PRED_SPW; si++) {
-    const uint32_t s = s0 + wave * PRED_SPW + si;
-    int32_t v = 0;
-    if (cvalid) {
-      if (lit == 0) v = (int32_t) L.tolerance;
-      else {
-        v = (int32_t)(lit < L.tolerance ? lit : L.tolerance) - (int32_t) acc[si];
-        if (v < 0) v = 0;
+    for (int i = 0; i < PRED_SPW; i++) acc[i] = 0;
+    for (uint32_t w0 = 0; w0 < L.wd; w0 += PRED_WC) {
+      __syncthreads();
+      uint32_t *xw = reinterpret_cast<uint32_t *>(&xs[0][0]);
+      const uint32_t wm = word_mask(w0 + wcol, L.wd, L.last_eval);
+#pragma unroll
+      for (uint32_t k = 0; k < STG; k++) xw[tid + k * 64 * PRED_WAVES] = ~pre[k] & wm;
+      __syncthreads();
+      if (w0 + PRED_WC < L.wd) fetch_slice(w0 + PRED_WC);
+      else if (ct + 1 < ct1) fetch_slice(0);
+      // the slice's include words: a word past the row re-reads the last
+      // one and meets a zero staged word; a clause past the last one reads
+      // the last clause's words and votes nothing
+      const uint32_t sw = min((uint32_t) PRED_WC, L.wd - w0);   // words of this slice
+      const __amdgpu_buffer_rsrc_t ars = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(L.actT + (size_t) w0 * L.clauses_total), (short) 0, (int)(sw * L.clauses_total * 4), 0x00020000);
+      const uint32_t cl4 = min(c, L.clauses_total - 1) * 4;
+      uint32_t a[PRED_WC];
+#pragma unroll
+      for (int ww = 0; ww < PRED_WC; ww++)
+        a[ww] = __builtin_amdgcn_raw_buffer_load_b32(ars, cl4, min((uint32_t) ww, sw - 1) * L.clauses_total * 4, 0);
+#pragma unroll
+      for (int si = 0; si < PRED_SPW; si++) {
+        const uint32_t sl = wave * PRED_SPW + si;
+        // two accumulators keep two independent popcount chains in flight
+        uint32_t t0 = acc[si], t1 = 0;
+#pragma unroll
+        for (int q = 0; q < PRED_WC / 4; q++) {
+          const uint4 xv = xs[sl][q];
+          t0 = bcnt_acc(a[4 * q + 0] & xv.x, t0);
+          t1 = bcnt_acc(a[4 * q + 1] & xv.y, t1);
+          t0 = bcnt_acc(a[4 * q + 2] & xv.z, t0);
+          t1 = bcnt_acc(a[4 * q + 3] & xv.w, t1);
+        }
+        acc[si] = t0 + t1;
       }
-      if (c & 1) v = -v;
     }
+
+    // votes (capi.c:110-119) and their class sums (capi.c:928-933); a vote is
+    // at most the tolerance (< 2^25, checked on the host), so a tile's 64
+    // votes sum exactly in 32 bits
+    const uint32_t lit = cvalid ? L.lit[c] : 0u;
+    const uint32_t cbase = ct * 64;
+    const uint32_t cls_first = cbase / L.clauses;
+    const uint32_t cls_last = min(cbase + 63, L.clauses_total - 1) / L.clauses;
+    const uint32_t my_cls = c / L.clauses;
+    // cap - failed, clamped at 0, signed by polarity; a clause without
+    // literals votes the tolerance (its failed count is 0)
+    const int32_t cap = !cvalid ? 0 : (int32_t)(lit == 0 ? L.tolerance : (lit < L.tolerance ? lit : L.tolerance));
+    const int32_t neg = -(int32_t)(c & 1);
     for (uint32_t k = cls_first; k <= cls_last; k++) {
-      int32_t t = (my_cls == k) ? v : 0;
+      if (k != rc) {
+        flush();
+        rc = k;
+        run = 0;
+      }
+      const bool mine_k = my_cls == k;
+      int32_t tile = 0;   // lane si: the tile's sum for sample si
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-      if (lane == 0 && s < L.n) atomicAdd(&L.sums[(size_t) s * L.sums_stride + L.class_offset + k], t);
+      for (int si = 0; si < PRED_SPW; si++) {
+        int32_t v = cap - (int32_t) acc[si];
+        v = v < 0 || !mine_k ? 0 : v;
+        const uint32_t t = group_sum<64>((uint32_t)((v ^ neg) - neg));
+        tile = lane == (uint32_t) si ? (int32_t) t : tile;
+      }
+      run += tile;
     }
   }
+  flush();
 }
 
 // argmax with strict '>' from -inf: ties go to the lowest class (capi.c:934-942).
-__global__ void argmax_kernel(const int32_t *sums, uint32_t n, uint32_t classes, int64_t *out)
+__global__ void argmax_kernel(const int64_t *sums, uint32_t n, uint32_t classes, int64_t *out)
 {
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const int32_t *r = sums + (size_t) s * classes;
+  const int64_t *r = sums + (size_t) s * classes;
   int64_t best = -INT64_MAX;
   uint32_t bc = 0;
   for (uint32_t k = 0; k < classes; k++)
-    if ((int64_t) r[k] > best) { best = r[k]; bc = k; }
+    if (r[k] > best) { best = r[k]; bc = k; }
   out[s] = bc;
 }
 
 // bit h = (sum_h > 0), LSB first; padding bits zero (capi.c:984-990).
-__global__ void encode_kernel(const int32_t *sums, uint32_t n, uint32_t classes, uint8_t *out)
+__global__ void encode_kernel(const int64_t *sums, uint32_t n, uint32_t classes, uint8_t *out)
 {
   const uint32_t cc = (classes + 7) / 8;
   const uint64_t total = (uint64_t) n * cc, stride = (uint64_t) gridDim.x * blockDim.x;
@@ -597,37 +640,42 @@ int tm_launch_predict(const tm_predict_launch &L, hipStream_t st)
 {
   const uint64_t spt = PRED_WAVES * PRED_SPW;            // samples per tile
   const uint64_t nct = (L.clauses_total + 63) / 64;
+  const uint64_t ngr = (nct + PRED_TPG - 1) / PRED_TPG;  // tile groups
   // A dispatch's grid is at most 2^32 - 1 work-items: larger sample counts
   // go out as several launches over consecutive sample ranges.
   const uint64_t max_blocks = 0xFFFFFFFFull / (64 * PRED_WAVES);
-  if (nct > max_blocks) return (int) hipErrorInvalidValue;
-  const uint64_t st_per_launch = max_blocks / nct;
+  // buffer descriptors: a slice of the include plane (32 words of every
+  // clause) and a tile's rows must each stay below 2^31 bytes
+  if (ngr > max_blocks || L.n == 0 || (uint64_t) L.clauses_total * PRED_WC * 4 > 0x7FFFFFFFull ||
+      spt * L.wd * 4 > 0x7FFFFFFFull)
+    return (int) hipErrorInvalidValue;
+  const uint64_t st_per_launch = max_blocks / ngr;
   tm_predict_launch P = L;
   P.sums_stride = L.sums_stride ? L.sums_stride : L.classes;
-  if (P.class_offset + L.classes > P.sums_stride) return (int) hipErrorInvalidValue;
-  // clause tiles per pass: about 2 MiB of include plane
-  const uint64_t tile_bytes = 64ull * L.wd * 4;
-  P.pass_tiles = (uint32_t) std::max<uint64_t>(1, std::min<uint64_t>(nct, (2ull << 20) / tile_bytes));
+  if (P.class_offset + L.classes > P.sums_stride || L.tolerance >= (1u << 25)) return (int) hipErrorInvalidValue;
+  // tile groups per pass: about 2 MiB of include plane
+  const uint64_t group_bytes = 64ull * PRED_TPG * L.wd * 4;
+  P.pass_groups = (uint32_t) std::max<uint64_t>(1, std::min<uint64_t>(ngr, (2ull << 20) / group_bytes));
   for (uint64_t s0 = 0; s0 < L.n; s0 += st_per_launch * spt) {
     const uint64_t n = std::min<uint64_t>(L.n - s0, st_per_launch * spt);
     P.rows = L.rows + s0 * L.wd;
     P.sums = L.sums + s0 * P.sums_stride;
     P.n = (uint32_t) n;
     const uint64_t nst = (n + spt - 1) / spt;
-    hipLaunchKernelGGL(predict_kernel, dim3((uint32_t)(nst * nct)), dim3(64 * PRED_WAVES), 0, st, P);
+    hipLaunchKernelGGL(predict_kernel, dim3((uint32_t)(nst * ngr)), dim3(64 * PRED_WAVES), 0, st, P);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int) e;
   }
   return (int) hipSuccess;
 }
 
-int tm_launch_argmax(const int32_t *sums, uint32_t n, uint32_t classes, int64_t *out, hipStream_t st)
+int tm_launch_argmax(const int64_t *sums, uint32_t n, uint32_t classes, int64_t *out, hipStream_t st)
 {
   hipLaunchKernelGGL(argmax_kernel, dim3((n + 255) / 256), dim3(256), 0, st, sums, n, classes, out);
   return (int) hipGetLastError();
 }
 
-int tm_launch_encode(const int32_t *sums, uint32_t n, uint32_t classes, uint8_t *out, hipStream_t st)
+int tm_launch_encode(const int64_t *sums, uint32_t n, uint32_t classes, uint8_t *out, hipStream_t st)
 {
   const uint64_t total = (uint64_t) n * ((classes + 7) / 8);
   const uint32_t blocks = (uint32_t) std::min<uint64_t>((total + 255) / 256, 1u << 16);

@@ -37,6 +37,26 @@ def _is_device_tensor(x):
     return t.__module__.startswith("torch") and t.__name__ == "Tensor" and x.is_cuda
 
 
+_TORCH_DTYPES = {np.uint8: "uint8", np.int64: "int64"}
+
+
+def _device_buffer(x, dtype, nbytes, what):
+    """Pointer of a device tensor after the checks the kernels rely on: a
+    contiguous GPU tensor of the right dtype holding at least nbytes.  A bad
+    buffer raises here, before any kernel could read past it."""
+    if not _is_device_tensor(x):
+        raise TsetlinError("%s: expected a GPU tensor (the other inputs of this call are on the GPU)" % what)
+    want = _TORCH_DTYPES[dtype]
+    if str(x.dtype) != "torch." + want:
+        raise TsetlinError("%s: expected dtype %s, got %s" % (what, want, x.dtype))
+    if not x.is_contiguous():
+        raise TsetlinError("%s: tensor must be contiguous" % what)
+    have = x.numel() * x.element_size()
+    if have < nbytes:
+        raise TsetlinError("%s: tensor holds %d bytes, the call reads %d" % (what, have, nbytes))
+    return x.data_ptr()
+
+
 def _host_buffer(x, dtype=np.uint8):
     """(array, pointer) for a host buffer; keeps the array alive."""
     if isinstance(x, (bytes, bytearray, memoryview)):
@@ -130,7 +150,12 @@ class Machine:
             _check(fn(self._handle(), n, rp, ra.nbytes, tp, dp, iters, cb, None))
             return
         if _is_device_tensor(rows):
-            rp, tp = rows.data_ptr(), tgt.data_ptr()
+            info = self.info()
+            rp = _device_buffer(rows, np.uint8, n * info["input_chunks"], "train: problems")
+            if enc:
+                tp = _device_buffer(tgt, np.uint8, n * info["class_chunks"], "train: codes")
+            else:
+                tp = _device_buffer(tgt, np.int64, n * 8, "train: solutions")
             fn = self._lib.tm_train_encoder_dev if enc else self._lib.tm_train_classifier_dev
         else:
             ra, rp = _host_buffer(rows)
@@ -150,13 +175,14 @@ class Machine:
         enc = self.type() == "encoder"
         if _is_device_tensor(problems):
             import torch
+            pp = _device_buffer(problems, np.uint8, n * self.info()["input_chunks"], "predict: problems")
             if enc:
                 cc = (self.info()["classes"] + 7) // 8
                 out = torch.empty((n, cc), dtype=torch.uint8, device=problems.device)
-                _check(self._lib.tm_predict_encoder_dev(self._handle(), problems.data_ptr(), n, out.data_ptr()))
+                _check(self._lib.tm_predict_encoder_dev(self._handle(), pp, n, out.data_ptr()))
             else:
                 out = torch.empty(n, dtype=torch.int64, device=problems.device)
-                _check(self._lib.tm_predict_classifier_dev(self._handle(), problems.data_ptr(), n, out.data_ptr()))
+                _check(self._lib.tm_predict_classifier_dev(self._handle(), pp, n, out.data_ptr()))
             return out
         pa, pp = _host_buffer(problems)
         if enc:
@@ -197,9 +223,15 @@ class Machine:
         _check(self._lib.tm_persist(self._handle(), buf.ctypes.data, ln.value, ctypes.byref(ln)))
         return buf.tobytes()
 
-    # capi.c:1564-1576: the buffer is resized by the callee
+    # capi.c:1564-1576: the buffer is resized by the callee; a GPU tensor of
+    # uint8 is filled in place (tm_checkpoint_dev)
     def checkpoint(self, buf=None):
         ln = ctypes.c_size_t(0)
+        if buf is not None and _is_device_tensor(buf):
+            size = self.info()["action_chunks"]
+            p = _device_buffer(buf, np.uint8, size, "checkpoint")
+            _check(self._lib.tm_checkpoint_dev(self._handle(), p, size, ctypes.byref(ln)))
+            return buf
         rc = self._lib.tm_checkpoint(self._handle(), None, 0, ctypes.byref(ln))
         if rc not in (_c.TM_OK, -5):
             _check(rc)
@@ -212,8 +244,23 @@ class Machine:
 
     # capi.c:1578-1588
     def restore(self, buf):
+        if _is_device_tensor(buf):
+            p = _device_buffer(buf, np.uint8, 0, "restore")
+            _check(self._lib.tm_restore_dev(self._handle(), p, buf.numel()))
+            return
         a, p = _host_buffer(buf)
         _check(self._lib.tm_restore(self._handle(), p, a.nbytes))
+
+    def counts(self):
+        """The counters in the reference layout counts[clause][plane][input_chunks]
+        (tm_counts; a diagnostic read, no reference counterpart)."""
+        ln = ctypes.c_size_t(0)
+        rc = self._lib.tm_counts(self._handle(), None, 0, ctypes.byref(ln))
+        if rc not in (_c.TM_OK, -5):
+            _check(rc)
+        out = np.zeros(ln.value, np.uint8)
+        _check(self._lib.tm_counts(self._handle(), out.ctypes.data, ln.value, ctypes.byref(ln)))
+        return out
 
     # capi.c:1590-1673
     def reconfigure(self, opts=None, **kw):

@@ -22,14 +22,15 @@ EXPORTS = [
     "tm_options_default", "tm_create_classifier", "tm_create_encoder", "tm_train_classifier",
     "tm_train_encoder", "tm_train_classifier_dev", "tm_train_encoder_dev", "tm_predict_classifier",
     "tm_predict_encoder", "tm_predict_classifier_dev", "tm_predict_encoder_dev", "tm_persist",
-    "tm_persist_file", "tm_load", "tm_load_file", "tm_checkpoint", "tm_restore", "tm_reconfigure",
+    "tm_persist_file", "tm_load", "tm_load_file", "tm_checkpoint", "tm_restore", "tm_checkpoint_dev",
+    "tm_restore_dev", "tm_counts", "tm_reconfigure",
     "tm_type", "tm_destroy", "tm_get_info", "tm_last_error", "tm_last_timing", "tm_last_train_stats",
     "tm_last_queue_timing",
     "tm_create_classifier_ind", "tm_create_encoder_ind", "tm_train_classifier_ind", "tm_train_encoder_ind",
     "tm_predict_classifier_ind", "tm_predict_encoder_ind", "tm_restrict", "tm_get_feature_sizes",
     "tm_bits_to_cvec_dev", "tm_flip_interleave_dev", "tm_class_accuracy_dev", "tm_encoding_accuracy_dev",
     "tm_batch_create", "tm_batch_machine", "tm_batch_train_classifier", "tm_batch_train_classifier_dev",
-    "tm_batch_accuracy_dev", "tm_batch_tuning", "tm_batch_timing", "tm_batch_destroy",
+    "tm_batch_accuracy_dev", "tm_batch_class_accuracy_dev", "tm_batch_tuning", "tm_batch_timing", "tm_batch_destroy",
 ]
 
 
@@ -98,6 +99,9 @@ def load_library(path=LIB_PATH):
         "tm_load_file": (ctypes.c_int, [ctypes.c_char_p, m_pp]),
         "tm_checkpoint": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, sz_p]),
         "tm_restore": (ctypes.c_int, [vp, u8p, ctypes.c_size_t]),
+        "tm_checkpoint_dev": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, sz_p]),
+        "tm_restore_dev": (ctypes.c_int, [vp, u8p, ctypes.c_size_t]),
+        "tm_counts": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, sz_p]),
         "tm_reconfigure": (ctypes.c_int, [vp, opt_p]),
         "tm_type": (ctypes.c_int, [vp]),
         "tm_destroy": (None, [vp]),
@@ -125,6 +129,7 @@ def load_library(path=LIB_PATH):
         "tm_batch_train_classifier": (ctypes.c_int, [vp, u32, u8p, i64p, u32, BATCH_EPOCH_FN, vp]),
         "tm_batch_train_classifier_dev": (ctypes.c_int, [vp, u32, u8p, i64p, u32, BATCH_EPOCH_FN, vp]),
         "tm_batch_accuracy_dev": (ctypes.c_int, [vp, u8p, i64p, u32, ctypes.POINTER(ctypes.c_double)]),
+        "tm_batch_class_accuracy_dev": (ctypes.c_int, [vp, u8p, i64p, u32, ctypes.POINTER(ctypes.c_double)]),
         "tm_batch_tuning": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                                            ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double)]),
         "tm_batch_timing": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),

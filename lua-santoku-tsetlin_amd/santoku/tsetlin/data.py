@@ -140,22 +140,25 @@ def bits_to_cvec_dev(set_bits, n, features, out=None):
     [0, n * features), which are skipped."""
     import ctypes
     import torch
+    from . import _device_buffer
     set_bits = set_bits.contiguous()
-    if set_bits.dtype != torch.int64:
-        raise TypeError("set_bits must be int64")
+    sp = _device_buffer(set_bits, np.int64, set_bits.numel() * 8, "bits_to_cvec: set_bits")
     if out is None:
         out = torch.empty((n, (2 * features + 7) // 8), dtype=torch.uint8, device=set_bits.device)
+    op = _device_buffer(out, np.uint8, n * ((2 * features + 7) // 8), "bits_to_cvec: out")
     bad = ctypes.c_uint64(0)
-    _dev_check(_dev_lib().tm_bits_to_cvec_dev(set_bits.data_ptr(), set_bits.numel(), n, features, out.data_ptr(),
-                                              ctypes.byref(bad)))
+    _dev_check(_dev_lib().tm_bits_to_cvec_dev(sp, set_bits.numel(), n, features, op, ctypes.byref(bad)))
     return out, bad.value
 
 
 def flip_interleave_dev(raw, n, features, out=None):
     """Flip-interleaved rows from bit-packed rows (ceil(F/8) bytes, LSB first)."""
     import torch
+    from . import _device_buffer
     raw = raw.contiguous()
+    rp = _device_buffer(raw, np.uint8, n * ((features + 7) // 8), "flip_interleave: raw")
     if out is None:
         out = torch.empty((n, (2 * features + 7) // 8), dtype=torch.uint8, device=raw.device)
-    _dev_check(_dev_lib().tm_flip_interleave_dev(raw.data_ptr(), n, features, out.data_ptr()))
+    op = _device_buffer(out, np.uint8, n * ((2 * features + 7) // 8), "flip_interleave: out")
+    _dev_check(_dev_lib().tm_flip_interleave_dev(rp, n, features, op))
     return out

@@ -44,8 +44,11 @@ def class_accuracy(predicted, expected, n_samples, n_classes, threads=None):
         from . import capi
         per = (ctypes.c_double * (3 * n_classes))()
         mac = (ctypes.c_double * 3)()
-        _dev_call(capi.load_library().tm_class_accuracy_dev, predicted.contiguous().data_ptr(),
-                  expected.contiguous().data_ptr(), n_samples, n_classes, per, mac)
+        from . import _device_buffer
+        predicted, expected = predicted.contiguous(), expected.contiguous()
+        pp = _device_buffer(predicted, np.int64, n_samples * 8, "class_accuracy: predicted")
+        ep = _device_buffer(expected, np.int64, n_samples * 8, "class_accuracy: expected")
+        _dev_call(capi.load_library().tm_class_accuracy_dev, pp, ep, n_samples, n_classes, per, mac)
         classes = [{"precision": per[3 * c], "recall": per[3 * c + 1], "f1": per[3 * c + 2]} for c in range(n_classes)]
         return {"classes": classes, "precision": mac[0], "recall": mac[1], "f1": mac[2]}
     p = _as_array(predicted, np.int64)[:n_samples].astype(np.uint64).astype(np.uint32)
@@ -80,8 +83,12 @@ def encoding_accuracy(predicted, expected, n_samples, n_hidden, threads=None):
         from . import capi
         bits = (ctypes.c_double * n_hidden)()
         summ = (ctypes.c_double * 4)()
-        _dev_call(capi.load_library().tm_encoding_accuracy_dev, predicted.contiguous().data_ptr(),
-                  expected.contiguous().data_ptr(), n_samples, n_hidden, bits, summ)
+        from . import _device_buffer
+        predicted, expected = predicted.contiguous(), expected.contiguous()
+        nb = n_samples * ((n_hidden + 7) // 8)
+        pp = _device_buffer(predicted, np.uint8, nb, "encoding_accuracy: predicted")
+        ep = _device_buffer(expected, np.uint8, nb, "encoding_accuracy: expected")
+        _dev_call(capi.load_library().tm_encoding_accuracy_dev, pp, ep, n_samples, n_hidden, bits, summ)
         return {"bits": list(bits), "mean_hamming": summ[0], "ber_min": summ[1], "ber_max": summ[2],
                 "ber_std": summ[3]}
     chunks = (n_hidden + 7) // 8

@@ -146,7 +146,7 @@ def features(params, ctx):
 class Proposer:
     """propose(samplers, n, history) for optimize.search_batched."""
 
-    def __init__(self, ctx, param_names=opt.PARAM_NAMES, seed=0, pool=4096, explore=0.25, min_history=24,
+    def __init__(self, ctx, param_names=opt.PARAM_NAMES, seed=0, pool=4096, explore=0.25, min_history=48,
                  steps=300, lr=3e-3, key=opt.make_key):
         self.ctx, self.names, self.pool, self.explore = dict(ctx), param_names, pool, explore
         self.min_history, self.steps, self.lr, self.key = min_history, steps, lr, key
@@ -208,6 +208,8 @@ class Proposer:
             picks.append(cand[i])
             if len(picks) >= n_model:
                 break
+        n_picked = len(picks)
         picks += plain[: n - len(picks)]
-        self.last = {"mode": "surrogate", "r2": float(r2), "loss": float(loss)}
+        # the first n_model proposals are the network's picks, the rest plain draws
+        self.last = {"mode": "surrogate", "r2": float(r2), "loss": float(loss), "n_model": n_picked}
         return picks

@@ -13,7 +13,7 @@ import sys
 
 import numpy as np
 
-from . import Machine, TsetlinError, _check, _host_buffer, _is_device_tensor, _options
+from . import Machine, TsetlinError, _check, _device_buffer, _host_buffer, _is_device_tensor, _options
 from . import capi as _c
 
 
@@ -57,8 +57,10 @@ class Batch:
 
         cb = _c.BATCH_EPOCH_FN(on_epoch) if each is not None else _c.BATCH_EPOCH_FN()
         if _is_device_tensor(problems):
-            _check(self._lib.tm_batch_train_classifier_dev(self._h, int(samples), problems.data_ptr(),
-                                                           solutions.data_ptr(), int(iterations), cb, None))
+            row = self._machines[0].info()["input_chunks"]
+            pp = _device_buffer(problems, np.uint8, int(samples) * row, "batch train: problems")
+            lp = _device_buffer(solutions, np.int64, int(samples) * 8, "batch train: solutions")
+            _check(self._lib.tm_batch_train_classifier_dev(self._h, int(samples), pp, lp, int(iterations), cb, None))
         else:
             ra, rp = _host_buffer(problems)
             la, lp = _host_buffer(solutions, np.int64)
@@ -68,8 +70,21 @@ class Batch:
     def accuracy(self, problems_dev, solutions_dev, n):
         """Held-out accuracy of every trial (device tensors)."""
         out = (ctypes.c_double * self.n)()
-        _check(self._lib.tm_batch_accuracy_dev(self._h, problems_dev.data_ptr(), solutions_dev.data_ptr(), int(n), out))
+        row = self._machines[0].info()["input_chunks"]
+        pp = _device_buffer(problems_dev, np.uint8, int(n) * row, "batch accuracy: problems")
+        lp = _device_buffer(solutions_dev, np.int64, int(n) * 8, "batch accuracy: solutions")
+        _check(self._lib.tm_batch_accuracy_dev(self._h, pp, lp, int(n), out))
         return np.array(out[:], dtype=np.float64)
+
+    def class_accuracy(self, problems_dev, solutions_dev, n):
+        """evaluator.class_accuracy of every trial (tm_batch_class_accuracy_dev):
+        an (ntrials, 3) array of macro precision, recall, F1."""
+        out = (ctypes.c_double * (3 * self.n))()
+        row = self._machines[0].info()["input_chunks"]
+        pp = _device_buffer(problems_dev, np.uint8, int(n) * row, "batch class accuracy: problems")
+        lp = _device_buffer(solutions_dev, np.int64, int(n) * 8, "batch class accuracy: solutions")
+        _check(self._lib.tm_batch_class_accuracy_dev(self._h, pp, lp, int(n), out))
+        return np.array(out[:], dtype=np.float64).reshape(self.n, 3)
 
     def tuning(self):
         lanes, wpl, order, ms = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_double()
@@ -95,26 +110,50 @@ class Batch:
 
 
 def batch_trial_fn(base, train_rows, train_y, dev_rows, dev_y, iterations, patience=None, tolerance=1e-8,
-                   seed=0):
-    """A batch_fn for optimize.search_batched: trains every trial of a round in
-    one Batch with per-epoch dev accuracy, patience early stop (a frozen trial
-    keeps the state of its last epoch; its score is its best epoch's accuracy,
-    optimize.lua:600-626) and returns the scores."""
+                   seed=0, metric="f1"):
+    """A batch_fn for optimize.search_batched: every trial of a round trains
+    in one Batch, scored after each epoch with the reference's search metric
+    (macro F1 of evaluator.class_accuracy on the dev rows, as the classifier
+    specs' search_metric, classify/mnist.lua:93-97; metric="accuracy" scores
+    plain accuracy).  As train_tm (optimize.lua:586-653): a trial that improves
+    by more than `tolerance` checkpoints its actions (kept in device memory),
+    `patience` epochs without improvement freeze it, and after training every
+    checkpointed trial is restored to its best epoch and scored again, which is
+    its result.  Without patience there is no checkpoint and a trial's result
+    is its last epoch's score.  Trial t takes seed + t."""
+    import torch
+
     from .optimize import _swap_tolerance
+
+    def score(b):
+        if metric == "accuracy":
+            return b.accuracy(dev_rows, dev_y, dev_y.shape[0])
+        return b.class_accuracy(dev_rows, dev_y, dev_y.shape[0])[:, 2]
 
     def fn(batch_params, info):
         trials = [dict(_swap_tolerance(dict(p)), seed=seed + i) for i, p in enumerate(batch_params)]
         b = Batch(base, trials)
-        best = np.full(len(trials), -np.inf)
-        since = np.zeros(len(trials), np.int64)
+        nt = len(trials)
+        best = np.full(nt, -np.inf)
+        last = np.full(nt, -np.inf)
+        since = np.zeros(nt, np.int64)
+        best_epoch = np.zeros(nt, np.int64)
+        ckpt = [None] * nt
 
         def each(epoch, active):
-            acc = b.accuracy(dev_rows, dev_y, dev_y.shape[0])
-            for t in range(len(trials)):
+            sc = score(b)
+            for t in range(nt):
                 if not active[t]:
                     continue
-                if acc[t] > best[t] + tolerance:
-                    best[t], since[t] = acc[t], 0
+                last[t] = sc[t]
+                if sc[t] > best[t] + tolerance:
+                    best[t], since[t], best_epoch[t] = sc[t], 0, epoch
+                    if patience:
+                        m = b.machine(t)
+                        if ckpt[t] is None:
+                            ckpt[t] = torch.empty(m.info()["action_chunks"], dtype=torch.uint8,
+                                                  device=dev_rows.device)
+                        m.checkpoint(ckpt[t])
                 else:
                     since[t] += 1
                 if patience and since[t] >= patience:
@@ -122,6 +161,16 @@ def batch_trial_fn(base, train_rows, train_y, dev_rows, dev_y, iterations, patie
             return True
 
         b.train(train_rows.shape[0], train_rows, train_y, iterations, each)
+        if any(c is not None for c in ckpt):
+            for t in range(nt):
+                if ckpt[t] is not None:
+                    b.machine(t).restore(ckpt[t])
+            sc = score(b)
+            for t in range(nt):
+                if ckpt[t] is not None:
+                    last[t] = sc[t]
         fn.last_batch = b
-        return [float(x) for x in best]
+        fn.last_best = best.copy()
+        fn.last_best_epoch = best_epoch.copy()
+        return [float(x) for x in last]
     return fn

@@ -6,6 +6,7 @@ import santoku.tsetlin as tm
 from oracle import oracle as O
 from santoku.tsetlin import data as D
 from santoku.tsetlin import trials as TR
+from state import assert_same_state
 
 pytestmark = pytest.mark.gpu
 
@@ -22,10 +23,16 @@ def trial_opts(rng, k):
     return out
 
 
-def solo(opts, rows, y, epochs):
-    m = tm.classifier(dict(BASE, **opts))
+def solo(opts, rows, y, epochs, base=BASE):
+    m = tm.classifier(dict(base, reusable=True, **opts))
     m.train(samples=rows.shape[0], problems=rows, solutions=y, iterations=epochs)
-    return m.checkpoint()
+    return m
+
+
+def assert_same_machine(a, b, label=""):
+    """Two engine machines hold the same actions and counters."""
+    assert np.array_equal(a.checkpoint(), b.checkpoint()), "%s: actions differ" % label
+    assert np.array_equal(a.counts(), b.counts()), "%s: count planes differ" % label
 
 
 def test_batch_trials_bitexact_vs_solo():
@@ -37,12 +44,11 @@ def test_batch_trials_bitexact_vs_solo():
     tune = b.tuning()
     assert tune["lanes"] * tune["words_per_lane"] >= 16
     for t, o in enumerate(trials):
-        got = b.machine(t).checkpoint()
-        assert np.array_equal(got, solo(o, d["train_rows"], d["train_y"], 3)), t
+        assert_same_machine(b.machine(t), solo(o, d["train_rows"], d["train_y"], 3), "trial %d" % t)
     # one trial against the oracle replay as well
     orc = O.Machine(O.params("classifier", **dict(BASE, **trials[2])))
     orc.train_replay(d["train_rows"], labels=d["train_y"], epochs=3)
-    assert np.array_equal(b.machine(2).checkpoint(), orc.actions)
+    assert_same_state(b.machine(2), orc, label="trial 2")
     assert np.array_equal(b.machine(2).predict(d["test_rows"], 300), orc.predict(d["test_rows"]))
 
 
@@ -61,8 +67,8 @@ def test_batch_freeze_equals_shorter_solo_and_accuracy():
 
     b.train(1200, d["train_rows"], d["train_y"], 6, each)
     assert [e for e, _ in seen] == [1, 2, 3, 4]
-    assert np.array_equal(b.machine(1).checkpoint(), solo(trials[1], d["train_rows"], d["train_y"], 2))
-    assert np.array_equal(b.machine(0).checkpoint(), solo(trials[0], d["train_rows"], d["train_y"], 4))
+    assert_same_machine(b.machine(1), solo(trials[1], d["train_rows"], d["train_y"], 2), "frozen trial")
+    assert_same_machine(b.machine(0), solo(trials[0], d["train_rows"], d["train_y"], 4), "trial 0")
     rows = torch.from_numpy(d["test_rows"]).cuda()
     ys = torch.from_numpy(d["test_y"]).cuda()
     acc = b.accuracy(rows, ys, 400)
@@ -89,3 +95,97 @@ def test_batch_device_inputs_and_search_round():
     fn = TR.batch_trial_fn(BASE, rows, ys, dev_rows, dev_y, iterations=3, patience=2)
     best, score, hist = opt.search_batched(opt.PARAM_NAMES, samplers, fn, rounds=2, trials=6, make_key=opt.make_key)
     assert len(hist) >= 6 and 0.0 <= score <= 1.0 and best is not None
+
+
+# ---------------------------------------------------------------------------
+# workload 4: 64 trials on workload 1's data (8192 x 256 features, 4 classes),
+# clauses log-uniform in 64..4096 per class, specificity uniform in 2..40,
+# target clauses/8 or clauses/4; tolerance and maximum (not fixed by the
+# workload) log-uniform in 1..64
+
+C4_SPACE = {"clauses": {"def": 512, "min": 64, "max": 4096, "log": True, "round": 8},
+            "clause_tolerance": {"def": 8, "min": 1, "max": 64, "log": True, "int": True},
+            "clause_maximum": {"def": 16, "min": 1, "max": 64, "log": True, "int": True},
+            "target": [0.125, 0.25],
+            "specificity": {"def": 10, "min": 2, "max": 40},
+            "include_bits": 1}
+
+
+def c4_trials(seed, k=64):
+    from santoku.tsetlin import optimize as opt
+    samplers = opt.build_samplers(C4_SPACE, opt.PARAM_NAMES, 1.0, np.random.default_rng(seed))
+    out = []
+    for i in range(k):
+        p = {n: (s.sample(None) if s.type == "range" else s.sample()) for n, s in samplers.items() if s is not None}
+        out.append(dict(opt._swap_tolerance(p), seed=5000 + i))
+    return out
+
+
+def test_workload4_sixty_four_trials_bitexact():
+    """All 64 trials of one batched launch per epoch equal their solo runs
+    (actions and every count plane), and four of them equal the oracle's
+    replay of chunks spread over their classes."""
+    import torch
+    d = D.xor_and_task(7)
+    rows = torch.from_numpy(d["train_rows"]).cuda()
+    ys = torch.from_numpy(d["train_y"]).cuda()
+    trials = c4_trials(3)
+    cl = [t["clauses"] for t in trials]
+    assert min(cl) < 128 and max(cl) > 2048, cl
+    b = TR.Batch(BASE, trials)
+    b.train(8192, rows, ys, 3)
+    for t, o in enumerate(trials):
+        assert_same_machine(b.machine(t), solo(o, rows, ys, 3), "trial %d %s" % (t, o))
+    order = np.argsort(cl)
+    for t in (int(order[0]), int(order[-1]), int(order[20]), int(order[45])):
+        o = dict(BASE, **trials[t])
+        orc = O.Machine(O.params("classifier", **o))
+        per = orc.p.clause_chunks
+        chunks = sorted({0, per - 1, per + per // 2, 3 * per + per // 3, 4 * per - 1})
+        orc.train_replay_chunks(d["train_rows"], chunks, labels=d["train_y"], epochs=3)
+        assert_same_state(b.machine(t), orc, chunks=chunks, label="trial %d %s" % (t, o))
+
+
+def test_batch_class_accuracy_is_the_evaluator():
+    """The batch scorer's macro precision / recall / F1 equal
+    evaluator.class_accuracy of each trial's predictions, double for double."""
+    import torch
+    from santoku.tsetlin import evaluator as E
+    d = D.xor_and_task(8, n_train=2000, n_test=700)
+    trials = trial_opts(np.random.default_rng(9), 5)
+    b = TR.Batch(BASE, trials)
+    b.train(2000, d["train_rows"], d["train_y"], 3)
+    rows = torch.from_numpy(d["test_rows"]).cuda()
+    ys = torch.from_numpy(d["test_y"]).cuda()
+    got = b.class_accuracy(rows, ys, 700)
+    for t in range(5):
+        want = E.class_accuracy(b.machine(t).predict(d["test_rows"], 700), d["test_y"], 700, 4)
+        assert (got[t, 0], got[t, 1], got[t, 2]) == (want["precision"], want["recall"], want["f1"]), t
+
+
+def test_batch_trial_fn_restores_best_epoch():
+    """optimize.lua:586-653 in the batched driver: with patience, a trial ends
+    on its best epoch's actions (restored from the device checkpoint) and its
+    result is that epoch's score; each such machine equals a solo run of that
+    many epochs."""
+    import torch
+    from santoku.tsetlin import optimize as opt
+    d = D.xor_and_task(14, n_train=3000, n_test=800)
+    rows = torch.from_numpy(d["train_rows"]).cuda()
+    ys = torch.from_numpy(d["train_y"]).cuda()
+    drows = torch.from_numpy(d["test_rows"]).cuda()
+    dys = torch.from_numpy(d["test_y"]).cuda()
+    trials = trial_opts(np.random.default_rng(15), 6)
+    params = [{k: v for k, v in t.items() if k != "seed"} for t in trials]
+    fn = TR.batch_trial_fn(BASE, rows, ys, drows, dys, iterations=8, patience=2, seed=300)
+    scores = fn(params, {"round": 1})
+    b = fn.last_batch
+    final = b.class_accuracy(drows, dys, 800)[:, 2]
+    assert np.array_equal(np.array(scores), final)
+    assert np.array_equal(final, fn.last_best)
+    for t in range(6):
+        e = int(fn.last_best_epoch[t])
+        assert 1 <= e <= 8
+        # the driver swaps an inverted tolerance / maximum (optimize.lua:731-733)
+        ref = solo(dict(opt._swap_tolerance(dict(params[t])), seed=300 + t), d["train_rows"], d["train_y"], e)
+        assert np.array_equal(b.machine(t).checkpoint(), ref.checkpoint()), (t, e)

@@ -12,6 +12,7 @@ import pytest
 import santoku.tsetlin as tm
 from oracle import oracle as O
 from santoku.tsetlin import data as D
+from state import assert_same_state
 
 pytestmark = pytest.mark.gpu
 
@@ -64,8 +65,7 @@ def test_c3_shape_train_bitexact_vs_replay(c3_machine):
     st = eng.train_stats()
     assert st["steps"] == 12 * 256 * 128 == int(orc.stats[0])
     assert st["draws"] > 0 and st["type1_false"] > 0
-    got = eng.checkpoint()
-    assert np.array_equal(got, orc.actions), "actions differ in %d bytes" % int((got != orc.actions).sum())
+    assert_same_state(eng, orc, label="workload-3 shape, 12 samples")
 
 
 def test_c3_full_size_predict_batch_independent(c3_machine):
@@ -106,8 +106,4 @@ def test_c3_shape_train_longer_vs_chunk_replay(c3_machine):
     orc.actions[:] = 0
     orc.train_replay_chunks(d["rows"], chunks, codes=d["codes"], epochs=2)
     assert orc.stats[1] > 0 and orc.stats[3] > 0
-    got = eng.checkpoint()
-    B = orc.p.input_chunks
-    for ch in chunks:
-        sl = slice(ch * 8 * B, (ch + 1) * 8 * B)
-        assert np.array_equal(got[sl], orc.actions[sl]), "chunk %d differs" % ch
+    assert_same_state(eng, orc, chunks=chunks, label="workload-3 shape, 2048 samples x 2 epochs")

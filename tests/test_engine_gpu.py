@@ -15,6 +15,7 @@ import pytest
 import santoku.tsetlin as tm
 from oracle import oracle as O
 from santoku.tsetlin import data as D
+from state import assert_same_state, count_mask
 
 pytestmark = pytest.mark.gpu
 
@@ -121,8 +122,7 @@ def test_train_bitexact_vs_replay(features, classes, clauses, kind, include_bits
         eng.train(samples=n, sentences=rows, codes=codes, iterations=epochs)
         orc.train_replay(rows, codes=codes, epochs=epochs)
     assert orc.stats[1] > 0 and orc.stats[3] > 0
-    got = eng.checkpoint()
-    assert np.array_equal(got, orc.actions), "actions differ in %d bytes" % int((got != orc.actions).sum())
+    assert_same_state(eng, orc)
     test_rows = rows_for(features, 200, rng, 0.3)
     assert np.array_equal(eng.predict(test_rows, 200), orc.predict(test_rows))
 
@@ -131,11 +131,11 @@ def test_train_workload1_bitexact():
     d = D.xor_and_task(2)
     o = dict(classes=4, features=256, clauses=256, clause_tolerance=64, clause_maximum=64, specificity=8.0,
              target=64, seed=99)
-    eng = tm.classifier(o)
+    eng = tm.classifier(dict(o, reusable=True))
     orc = O.Machine(O.params("classifier", **o))
     eng.train(samples=8192, problems=d["train_rows"], solutions=d["train_y"], iterations=3)
     orc.train_replay(d["train_rows"], labels=d["train_y"], epochs=3)
-    assert np.array_equal(eng.checkpoint(), orc.actions)
+    assert_same_state(eng, orc)
     assert np.array_equal(eng.predict(d["test_rows"], 2048), orc.predict(d["test_rows"]))
 
 
@@ -155,7 +155,7 @@ def test_repeat_hits_bitexact(features, kind, state):
     n = 160
     o = dict(classes=4, features=features, clauses=16, clause_tolerance=6, clause_maximum=8, specificity=1.0,
              target=6, seed=4321, state=state)
-    eng = tm.create(kind, o)
+    eng = tm.create(kind, dict(o, reusable=True))
     orc = O.Machine(O.params(kind, **o))
     rows = rows_for(features, n, rng, 0.3)
     if kind == "classifier":
@@ -168,7 +168,7 @@ def test_repeat_hits_bitexact(features, kind, state):
         orc.train_replay(rows, codes=codes, epochs=3)
     # oracle counters: steps, Type I true, Type II, Type I false, draws, skips
     assert orc.stats[3] > 0 and orc.stats[4] == orc.stats[3] * 2 * features
-    assert np.array_equal(eng.checkpoint(), orc.actions)
+    assert_same_state(eng, orc)
     st = eng.train_stats()
     assert st["type1_false"] == int(orc.stats[3]) and st["draws"] == int(orc.stats[4])
 
@@ -182,7 +182,7 @@ def test_negative_sampling_variants():
         eng, orc = pair(features=40, classes=4, clauses=16, negative=neg, seed=3, target=6)
         eng.train(samples=300, problems=rows, solutions=labels, iterations=2)
         orc.train_replay(rows, labels=labels, epochs=2)
-        assert np.array_equal(eng.checkpoint(), orc.actions), neg
+        assert_same_state(eng, orc, label="negative %s" % neg)
 
 
 QUEUE_SHAPES = [
@@ -211,12 +211,15 @@ def test_step_queue_matches_scan(features, classes, clauses, neg, n, monkeypatch
         monkeypatch.setenv("TSETLIN_AMD_QUEUE", mode)
         eng, orc = pair(**o)
         eng.train(samples=n, problems=rows, solutions=labels, iterations=3)
-        got[mode] = (eng.checkpoint(), eng.train_stats(), eng.queue_timing())
+        got[mode] = (eng.checkpoint(), eng.train_stats(), eng.queue_timing(), eng.counts())
     assert np.array_equal(got["1"][0], got["0"][0])
+    assert np.array_equal(got["1"][3], got["0"][3])
     assert got["1"][1] == got["0"][1]
     assert got["1"][2] > 0 and got["0"][2] == 0
     orc.train_replay(rows, labels=labels, epochs=3)
     assert np.array_equal(got["1"][0], orc.actions)
+    mask = count_mask(orc)
+    assert np.array_equal(got["1"][3] & mask, orc.counts & mask)
     assert got["1"][1]["steps"] == int(orc.stats[0])
 
 
@@ -235,7 +238,7 @@ def test_vote_width_at_large_tolerance(tolerance):
     eng, orc = pair(**o)
     eng.train(samples=n, problems=rows, solutions=labels, iterations=2)
     orc.train_replay(rows, labels=labels, epochs=2)
-    assert np.array_equal(eng.checkpoint(), orc.actions)
+    assert_same_state(eng, orc)
     assert eng.train_stats()["steps"] == int(orc.stats[0])
 
 
@@ -258,14 +261,13 @@ def test_every_tile_gives_the_same_machine(kind, monkeypatch):
         else:
             codes = np.random.default_rng(5).integers(0, 8, (n, 1), dtype=np.uint8)
             eng.train(samples=n, sentences=rows, codes=codes, iterations=2)
-        got = eng.checkpoint()
         if tile == "16x2":
             if kind == "classifier":
                 orc.train_replay(rows, labels=labels, epochs=2)
             else:
                 orc.train_replay(rows, codes=codes, epochs=2)
-            want = orc.actions.copy()
-        assert np.array_equal(got, want), tile
+            ref = orc
+        assert_same_state(eng, ref, label=tile)
 
 
 def test_each_callback_epochs_and_stop():
@@ -284,12 +286,12 @@ def test_each_callback_epochs_and_stop():
     eng.train(samples=200, problems=rows, solutions=labels, iterations=10, each=each)
     assert seen == [1, 2, 3]
     orc.train_replay(rows, labels=labels, epochs=3)
-    assert np.array_equal(eng.checkpoint(), orc.actions)
+    assert_same_state(eng, orc)
     assert np.array_equal(preds[-1], orc.predict(rows))
     # the callback path (one launch per epoch) equals one launch over all epochs
     eng2, _ = pair(features=64, classes=3, clauses=16, seed=8)
     eng2.train(samples=200, problems=rows, solutions=labels, iterations=3)
-    assert np.array_equal(eng2.checkpoint(), orc.actions)
+    assert_same_state(eng2, orc)
 
 
 def test_each_callback_error_stops_training(capsys):
@@ -317,7 +319,7 @@ def test_train_restarts_from_setup_each_call():
     eng.train(samples=100, problems=rows, solutions=labels, iterations=2)
     eng.train(samples=100, problems=rows, solutions=labels, iterations=2)
     orc.train_replay(rows, labels=labels, epochs=2)
-    assert np.array_equal(eng.checkpoint(), orc.actions)
+    assert_same_state(eng, orc)
 
 
 def test_device_resident_training():
@@ -329,7 +331,7 @@ def test_device_resident_training():
     eng.train(samples=500, problems=torch.from_numpy(rows).cuda(), solutions=torch.from_numpy(labels).cuda(),
               iterations=2)
     orc.train_replay(rows, labels=labels, epochs=2)
-    assert np.array_equal(eng.checkpoint(), orc.actions)
+    assert_same_state(eng, orc)
 
 
 # ---------------------------------------------------------------------------
@@ -413,7 +415,7 @@ def test_reconfigure_matches_fresh_machine():
     orc = O.Machine(O.params("classifier", classes=3, features=48, clauses=20, clause_tolerance=4,
                              clause_maximum=6, specificity=3.0, target=0.25, seed=6))
     orc.train_replay(rows, labels=labels, epochs=2)
-    assert np.array_equal(eng.checkpoint(), orc.actions)
+    assert_same_state(eng, orc)
 
 
 def test_zero_iterations_runs_setup():

@@ -14,6 +14,7 @@ import pytest
 import santoku.tsetlin as tm
 from oracle import oracle as O
 from santoku.tsetlin import data as D
+from state import assert_same_state
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +42,8 @@ def offsets_of(widths):
 
 
 def pair(kind, widths, **kw):
-    o = dict(clauses=16, clause_tolerance=6, clause_maximum=8, specificity=3.0, target=8, seed=77)
+    # reusable by default: the machine keeps its counters for the state checks
+    o = dict(clauses=16, clause_tolerance=6, clause_maximum=8, specificity=3.0, target=8, seed=77, reusable=True)
     o.update(kw)
     K = len(widths)
     eng = tm.create(kind, dict(o, classes=K, individualized=True, feat_offsets=offsets_of(widths)))
@@ -80,9 +82,7 @@ def test_ind_train_bitexact_vs_replay(widths, kind, n, epochs, negative):
     prob, offs = ragged(widths, n, rng, gap=3)
     train_both(eng, orc, kind, prob, offs, n, rng, epochs)
     assert orc.stats[1] > 0 and orc.stats[3] > 0
-    got = eng.checkpoint()
-    assert got.size == orc.actions.size
-    assert np.array_equal(got, orc.actions), "actions differ in %d bytes" % int((got != orc.actions).sum())
+    assert_same_state(eng, orc, label=str(widths))
     tprob, toffs = ragged(widths, 120, rng)
     assert np.array_equal(eng.predict(tprob, toffs, 120), orc.predict(tprob, toffs, 120))
 
@@ -102,8 +102,8 @@ def test_ind_epoch_callback_matches_grouped_epochs():
     b.train(samples=n, problems=prob, solutions=labels, dim_offsets=offs, iterations=10)
     orc.train_replay(prob, offs, n, labels=labels, epochs=10)
     assert seen == list(range(1, 11))
-    assert np.array_equal(a.checkpoint(), orc.actions)
-    assert np.array_equal(b.checkpoint(), orc.actions)
+    assert_same_state(a, orc, label="per-epoch launches")
+    assert_same_state(b, orc, label="grouped epochs")
 
 
 @pytest.mark.parametrize("kind", ["classifier", "encoder"])
@@ -187,13 +187,17 @@ def test_ind_restrict():
     eng, orc = pair("classifier", widths, reusable=True)
     prob, offs = ragged(widths, n, rng)
     train_both(eng, orc, "classifier", prob, offs, n, rng, 2)
-    full = eng.checkpoint()
+    assert_same_state(eng, orc)
+    full, full_counts = eng.checkpoint(), eng.counts()
     sl = orc.action_slices()
     sel = [3, 0, 2]
     eng.restrict(sel)
     info = eng.info()
     assert info["classes"] == 3 and eng.feature_sizes() == [widths[i] for i in sel]
     assert np.array_equal(eng.checkpoint(), np.concatenate([full[sl[i]] for i in sel]))
+    # the selected classes' counters move with them (7 planes per action byte)
+    csl = [slice(7 * x.start, 7 * x.stop) for x in sl]
+    assert np.array_equal(eng.counts(), np.concatenate([full_counts[csl[i]] for i in sel]))
     # restrict keeps the machine's negative sampling rate, 1/4 from its
     # original 4 classes (capi.c:752-900 leaves tm->negative alone)
     sub = O.IndMachine(O.params("classifier", classes=3, features=1, clauses=16, clause_tolerance=6,
@@ -205,7 +209,7 @@ def test_ind_restrict():
     labels = rng.integers(0, 3, n).astype(np.int64)
     eng.train(samples=n, problems=prob, solutions=labels, dim_offsets=offs[sel], iterations=2)
     sub.train_replay(prob, offs[sel], n, labels=labels, epochs=2)
-    assert np.array_equal(eng.checkpoint(), sub.actions)
+    assert_same_state(eng, sub, label="after restrict")
     for bad, msg in (([], "selected_dims cannot be empty"), ([0, 3], r"selected_dims\[1\] = 3 out of range \[0, 3\)"),
                      ([1, 2, 1], "duplicate dimension 1 in selected_dims"), ([-1], "out of range")):
         with pytest.raises(tm.TsetlinError, match=msg):
@@ -229,7 +233,7 @@ def test_ind_reconfigure_then_train():
     orc = O.IndMachine(O.params("encoder", classes=2, features=1, clauses=40, clause_tolerance=4, clause_maximum=5,
                                 specificity=2.0, target=5, seed=8), widths)
     orc.train_replay(prob, offs, n, codes=codes, epochs=2)
-    assert np.array_equal(eng.checkpoint(), orc.actions)
+    assert_same_state(eng, orc, label="after reconfigure")
 
 
 def test_ind_argument_errors():

@@ -399,3 +399,34 @@ def test_chunk_replay_equals_whole_replay(kind):
             assert np.array_equal(part.actions[sl], whole.actions[sl]), ch
         else:
             assert (part.actions[sl] == 0xEE).all(), ch
+
+
+@pytest.mark.parametrize("features,state", [(5, 8), (13, 4), (101, 8)])
+def test_padding_counters_reach_nothing(features, state):
+    """In a row's last byte on the reference's masked byte path
+    (input_chunks % 16 != 0), the padding bits' counters only count: the
+    carry into the action plane is masked (automata.h:97,106).  Scrambling
+    them between epochs leaves every action and every other counter of the
+    replay unchanged, so the GPU parity tests may leave them out."""
+    rng = np.random.default_rng(features)
+    o = dict(classes=3, features=features, clauses=16, clause_tolerance=4, clause_maximum=6, specificity=2.0,
+             target=6, seed=9, state=state, include_bits=2)
+    n = 120
+    from santoku.tsetlin import data as D
+    rows = D.bits_to_cvec(rng.random((n, features)) < 0.4)
+    y = rng.integers(0, 3, n).astype(np.int64)
+    a = O.Machine(O.params("classifier", **o))
+    b = O.Machine(O.params("classifier", **o))
+    assert a.p.input_chunks % 16 != 0 and a.p.tail_mask != 0xFF
+    a.train_replay(rows, labels=y, epochs=1)
+    b.train_replay(rows, labels=y, epochs=1)
+    B, M = a.p.input_chunks, state - 1
+    pad = np.uint8(~a.p.tail_mask & 0xFF)
+    cb = b.counts.reshape(-1, M, B)
+    cb[:, :, -1] ^= rng.integers(0, 256, cb[:, :, -1].shape, dtype=np.uint8) & pad
+    a.train_replay(rows, labels=y, epochs=2, epoch_first=1)
+    b.train_replay(rows, labels=y, epochs=2, epoch_first=1)
+    assert np.array_equal(a.actions, b.actions)
+    keep = np.tile(np.r_[np.full(B - 1, 0xFF, np.uint8), np.uint8(a.p.tail_mask)], a.nclauses * M)
+    assert np.array_equal(a.counts & keep, b.counts & keep)
+    assert not np.array_equal(a.counts, b.counts)

@@ -14,6 +14,7 @@ import santoku.tsetlin as tm
 from oracle import oracle as O
 from santoku.tsetlin import data as D
 from santoku.tsetlin import trials as TR
+from state import assert_same_state
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +42,7 @@ def test_train_bitexact_vs_replay_state_bits(state, features, classes, clauses, 
     spec = max(2.0, features / 16.0)
     o = dict(classes=classes, features=features, clauses=clauses, clause_tolerance=6, clause_maximum=8,
              specificity=spec, target=8, include_bits=include_bits, seed=4321, state=state)
-    eng = tm.create(kind, o)
+    eng = tm.create(kind, dict(o, reusable=True))
     orc = O.Machine(O.params(kind, **o))
     rows = rows_for(features, n, rng)
     if kind == "classifier":
@@ -55,8 +56,7 @@ def test_train_bitexact_vs_replay_state_bits(state, features, classes, clauses, 
     assert orc.stats[1] > 0 and orc.stats[3] > 0
     assert eng.info()["state_bits"] == state
     assert eng.info()["state_chunks"] == classes * orc.p.clauses * (state - 1) * orc.p.input_chunks
-    got = eng.checkpoint()
-    assert np.array_equal(got, orc.actions), "actions differ in %d bytes" % int((got != orc.actions).sum())
+    assert_same_state(eng, orc, label="state %d" % state)
     test = rows_for(features, 150, rng)
     assert np.array_equal(eng.predict(test, 150), orc.predict(test))
 
@@ -75,11 +75,11 @@ def test_state_bits_persist_and_every_tile():
     for tile in ("8x2", "16x1", "16x2", "32x1", "32x2", "64x1"):
         os.environ["TSETLIN_AMD_TILE"] = tile
         try:
-            eng = tm.classifier(o)
+            eng = tm.classifier(dict(o, reusable=True))
             eng.train(samples=160, problems=rows, solutions=labels, iterations=2)
         finally:
             del os.environ["TSETLIN_AMD_TILE"]
-        assert np.array_equal(eng.checkpoint(), orc.actions), tile
+        assert_same_state(eng, orc, label=tile)
     blob = eng.persist()
     h = struct.unpack("<i13I", blob[:56])
     assert h[7] == 6 and h[12] == 3 * 16 * 5 * 50 and h[13] == 3 * 16 * 50
@@ -99,13 +99,13 @@ def test_state_bits_individualized_and_batch():
         o_ += b.size
     prob, offs = np.concatenate(parts), np.array(offs, np.int64)
     o = dict(clauses=16, clause_tolerance=6, clause_maximum=8, specificity=3.0, target=8, seed=21, state=4)
-    eng = tm.create("classifier", dict(o, classes=3, individualized=True,
+    eng = tm.create("classifier", dict(o, classes=3, individualized=True, reusable=True,
                                        feat_offsets=np.concatenate([[0], np.cumsum(widths)])))
     orc = O.IndMachine(O.params("classifier", classes=3, features=1, **o), widths)
     labels = rng.integers(0, 3, n).astype(np.int64)
     eng.train(samples=n, problems=prob, solutions=labels, dim_offsets=offs, iterations=2)
     orc.train_replay(prob, offs, n, labels=labels, epochs=2)
-    assert np.array_equal(eng.checkpoint(), orc.actions)
+    assert_same_state(eng, orc, label="individualized, 4 state bits")
     # batched trials at 11 state bits equal their solo machines
     rows = rows_for(40, 200, rng)
     y = rng.integers(0, 4, 200).astype(np.int64)
@@ -117,7 +117,7 @@ def test_state_bits_individualized_and_batch():
     for i, t in enumerate(trials):
         ref = O.Machine(O.params("classifier", **dict(base, **t)))
         ref.train_replay(rows, labels=y, epochs=3)
-        assert np.array_equal(b.machine(i).checkpoint(), ref.actions), i
+        assert_same_state(b.machine(i), ref, label="batch trial %d" % i)
 
 
 def test_state_bits_limits():

@@ -86,3 +86,66 @@ def test_surrogate_gains_over_plain_search():
     # trials proposed in rounds 4-6 score higher on average, and the best found is no worse
     assert np.mean([m for _, m in sur]) > np.mean([m for _, m in plain]) + 0.01
     assert np.mean([b for b, _ in sur]) >= np.mean([b for b, _ in plain]) - 0.002
+
+
+# ---------------------------------------------------------------------------
+# workload 5: the surrogate fitted on outcomes the engine generates (batched
+# trials on workload 1's data, scored by the reference's search metric), against
+# the plain search (optimize.lua:391-516) over the workload-4 space
+
+def _engine_search(seed, surrogate, dev, rounds=5, trials=32, epochs=4):
+    """A search over the workload-4 space scored by the engine's batched
+    trials; returns the best score, and per surrogate round the mean scores of
+    the network's picks and of the plain draws proposed beside them."""
+    from santoku.tsetlin import trials as TR
+    from test_batch_gpu import BASE, C4_SPACE
+    d = dev
+    fn = TR.batch_trial_fn(BASE, d["rows"], d["ys"], d["drows"], d["dys"], iterations=epochs, patience=2,
+                           seed=seed * 1000)
+    samplers = opt.build_samplers(C4_SPACE, opt.PARAM_NAMES, 0.3, np.random.default_rng(seed))
+    prop = S.Proposer({"epochs": epochs, "density": 0.5, "classes": 4, "features": 256}, seed=seed,
+                      min_history=48) if surrogate else None
+    rounds_out, cur = [], []
+
+    def each(ev):
+        if ev["event"] == "trial":
+            cur.append(ev["score"])
+        elif ev["event"] == "round":
+            last = prop.last if prop is not None else {"mode": "plain"}
+            if last.get("mode") == "surrogate":
+                k = last["n_model"]
+                rounds_out.append((float(np.mean(cur[:k])), float(np.mean(cur[k:])) if cur[k:] else None))
+            else:
+                rounds_out.append(None)
+            cur.clear()
+
+    _, best, hist = opt.search_batched(opt.PARAM_NAMES, samplers, fn, rounds=rounds, trials=trials, propose=prop,
+                                       make_key=opt.make_key, each=each)
+    return best, rounds_out
+
+
+@pytest.fixture(scope="module")
+def workload1_dev():
+    import torch
+    from santoku.tsetlin import data as D
+    d = D.xor_and_task(21)
+    return {"rows": torch.from_numpy(d["train_rows"]).cuda(), "ys": torch.from_numpy(d["train_y"]).cuda(),
+            "drows": torch.from_numpy(d["test_rows"]).cuda(), "dys": torch.from_numpy(d["test_y"]).cuda()}
+
+
+def test_surrogate_on_engine_outcomes_gains_over_plain(workload1_dev):
+    """The network, fitted on the engine's own trial outcomes, picks trials
+    that score higher than the plain draws proposed beside them in the same
+    rounds (same sampler state, same data), and the surrogate-guided search
+    finds a better best than the plain search from the same seed."""
+    plain = [_engine_search(s, False, workload1_dev) for s in range(3)]
+    sur = [_engine_search(s, True, workload1_dev) for s in range(3)]
+    gains = [m - p for _, rs in sur for r in rs if r is not None and r[1] is not None for m, p in [r]]
+    print("plain best:", [round(b, 4) for b, _ in plain])
+    print("surrogate best:", [round(b, 4) for b, _ in sur])
+    print("surrogate rounds (picks, plain beside):", [rs for _, rs in sur])
+    print("per-round gain of the picks: mean %.4f over %d rounds" % (np.mean(gains), len(gains)))
+    # rounds 3-5 of 3 seeds, minus any that fell back to plain search
+    assert len(gains) >= 6, sur
+    assert np.mean(gains) > 0.01, gains
+    assert np.mean([b for b, _ in sur]) > np.mean([b for b, _ in plain]) + 0.005, (plain, sur)
