@@ -2,15 +2,23 @@
 
 The engine's random stream is its own (the reference's RNG lives in an
 unvendored dependency), so training trajectories differ from the reference's
-and parity is statistical: mean held-out accuracy over 5 seeds.  The CPU side
-is the oracle's reference-shaped trainer (per-thread PCG32 streams and
-Fisher-Yates shuffles, OpenMP static over chunks, evaluation before the
-negative-sampling test, as capi.c:1112-1191).
+and parity is statistical.  The CPU side is the oracle's reference-shaped
+trainer (per-thread PCG32 streams and Fisher-Yates shuffles, OpenMP static
+over chunks, evaluation before the negative-sampling test, as
+capi.c:1112-1191).
 
 Workload 1: 8192 train / 2048 test samples, 256 Bernoulli(0.5) features,
 class = 2*(x3 AND x17) + (x40 XOR x91) with 5% flips; 256 clauses/class,
-target 64, specificity 8, 10 epochs.
+target 64, specificity 8, 10 epochs.  At that configuration neither trainer
+learns past the majority class (about 0.37): clauses keep growing while they
+hold at most clause_maximum = 64 literals and end with about 109 included
+literals, against the 2-4 of the planted patterns, so they rarely match.  The
+parity that can fail is measured where both learn
+(test_parity_where_both_learn).
 """
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 
@@ -39,19 +47,57 @@ def accuracies(o, seeds, epochs=10):
 
 def test_workload1_accuracy_within_half_point():
     """The pinned configuration (clause tolerance = maximum = 64, the spec
-    defaults): mean held-out accuracy over 5 seeds within +-0.5 pt."""
+    defaults): mean held-out accuracy over 5 seeds within +-0.5 pt.  Both
+    trainers stay near the majority class here, so this checks that neither
+    does worse, not that they learn alike."""
     gpu, cpu = accuracies(dict(W1, clause_tolerance=64, clause_maximum=64), range(5))
     assert abs(gpu.mean() - cpu.mean()) <= 0.005, (gpu, cpu)
 
 
-def test_learning_setting_agrees_with_reference_shape():
-    """A setting where both trainers learn past the majority class (0.37):
-    tolerance = maximum = 3.  Per-seed spread is about 2 pt here, so the 5-seed
-    means are held to 1.5 pt (a CPU sweep of six such settings gave
-    differences from -1.2 to +0.5 pt, of both signs)."""
-    gpu, cpu = accuracies(dict(W1, clause_tolerance=3, clause_maximum=3, specificity=8.0, target=16), range(5))
-    assert gpu.mean() > 0.45 and cpu.mean() > 0.45, (gpu, cpu)
-    assert abs(gpu.mean() - cpu.mean()) <= 0.015, (gpu, cpu)
+# A setting on workload 1's data where both trainers learn (about 0.84 held
+# out, majority 0.37, ceiling 0.95), found by a CPU sweep of tolerance,
+# maximum, target and specificity: one literal per clause at most (tolerance
+# = maximum = 1), four random-literal draws per event (specificity 128), a
+# chunk vote clamp of 2, 20 epochs.  One run's held-out accuracy spreads by
+# about 0.9 pt, so 48 runs a side put the standard error of the difference at
+# about 0.25 pt: a 1-pt bias fails the 0.5-pt bound about 97 times in 100.
+LEARN = dict(classes=4, features=256, clauses=256, clause_tolerance=1, clause_maximum=1, target=2,
+             specificity=128.0)
+LEARN_EPOCHS = 20
+
+
+def test_parity_where_both_learn():
+    """48 GPU trainings against 48 reference-shaped CPU trainings (2 datasets
+    x 24 seeds; held-out sets of 16384 rows): mean held-out accuracy within
+    +-0.5 pt."""
+    gpu, cpu = [], []
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    for data_seed in (0, 1):
+        d = D.xor_and_task(data_seed, n_test=16384)
+        seeds = range(100 * data_seed, 100 * data_seed + 24)
+        for seed in seeds:
+            eng = tm.classifier(dict(LEARN, seed=seed))
+            eng.train(samples=8192, problems=d["train_rows"], solutions=d["train_y"], iterations=LEARN_EPOCHS)
+            gpu.append(float((eng.predict(d["test_rows"], 16384) == d["test_y"]).mean()))
+
+        def ref(seed):
+            m = O.Machine(O.params("classifier", **dict(LEARN, seed=seed)))
+            # one thread per run (the trajectory depends on the thread count;
+            # its distribution does not); runs in parallel
+            m.train_refshape(d["train_rows"], labels=d["train_y"], epochs=LEARN_EPOCHS, threads=1)
+            return float((m.predict(d["test_rows"]) == d["test_y"]).mean())
+
+        with ThreadPoolExecutor(workers) as ex:
+            cpu += list(ex.map(ref, seeds))
+    gpu, cpu = np.array(gpu), np.array(cpu)
+    se = np.sqrt(gpu.var(ddof=1) / gpu.size + cpu.var(ddof=1) / cpu.size)
+    print("GPU accuracies:", np.round(gpu, 4).tolist())
+    print("CPU accuracies:", np.round(cpu, 4).tolist())
+    print("means %.4f %.4f difference %.4f (standard error %.4f)" % (gpu.mean(), cpu.mean(),
+                                                                      gpu.mean() - cpu.mean(), se))
+    assert gpu.mean() > 0.8 and cpu.mean() > 0.8, (gpu.mean(), cpu.mean())
+    assert se < 0.0035, se
+    assert abs(gpu.mean() - cpu.mean()) <= 0.005, (gpu.mean(), cpu.mean())
 
 
 def test_encoder_hamming_within_half_point():

@@ -145,6 +145,52 @@ def test_mlp_oracle_adam_reduces_loss():
     assert MO.loss(p1, X, Y) < 0.3 * MO.loss(p0, X, Y)
 
 
+def test_mlp_oracle_adam_first_step_closed_form():
+    """From zero moments Adam's first step is lr * g / (|g| + eps) per
+    parameter after both bias corrections (Kingma & Ba 2015, Alg. 1); a
+    missing correction scales it by 0.1 or 1/sqrt(0.001).  The moments after
+    it are 0.1 g and 0.001 g^2, which catches swapped betas (their corrected
+    first step is the same)."""
+    from oracle import mlp_oracle as MO
+    rng = np.random.default_rng(10)
+    X = rng.standard_normal((50, 12))
+    Y = rng.standard_normal(50)
+    p0 = MO.init(5).astype(np.float64) + rng.standard_normal(MO.NPARAMS) * 0.01
+    g = MO.gradient(p0, X, Y)
+    lr = 1e-3
+    p1, m, v = MO.adam(p0, X, Y, 1, lr)
+    assert np.allclose(p0 - p1, lr * g / (np.abs(g) + 1e-8), rtol=1e-9, atol=1e-15)
+    assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.001 * g * g)
+
+
+def test_mlp_oracle_adam_matches_torch():
+    """Five steps of the oracle's Adam against torch.optim.Adam (float64, the
+    same initial parameters and full-batch mean squared error)."""
+    torch = pytest.importorskip("torch")
+    from oracle import mlp_oracle as MO
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((60, 12))
+    Y = np.sin(X[:, 0]) + 0.3 * X[:, 1]
+    p0 = MO.init(6).astype(np.float64)
+    w = MO.unpack(p0.copy())
+    params = [torch.tensor(w[name], dtype=torch.float64, requires_grad=True) for name, _ in MO.SIZES]
+    opt_ = torch.optim.Adam(params, lr=3e-3, betas=(0.9, 0.999), eps=1e-8)
+    Xt, Yt = torch.tensor(X), torch.tensor(Y)
+    for _ in range(5):
+        W1, b1, W2, b2, W3, b3, W4, b4 = params
+        a1 = torch.relu(Xt @ W1.T + b1)
+        a2 = torch.relu(a1 @ W2.T + b2)
+        a3 = torch.relu(a2 @ W3.T + b3)
+        out = a3 @ W4 + b4[0]
+        loss = torch.mean((out - Yt) ** 2)
+        opt_.zero_grad()
+        loss.backward()
+        opt_.step()
+    want = np.concatenate([q.detach().numpy().reshape(-1) for q in params])
+    got, _, _ = MO.adam(p0, X, Y, 5, 3e-3)
+    assert np.allclose(got, want, rtol=1e-10, atol=1e-12), np.abs(got - want).max()
+
+
 def test_surrogate_features_and_target():
     from santoku.tsetlin import surrogate as S
     p = dict(clauses=2048, clause_tolerance=64, clause_maximum=64, target=0.125, specificity=10.0, include_bits=1)
