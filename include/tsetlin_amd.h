@@ -308,7 +308,8 @@ int tm_encoding_accuracy_dev(const uint8_t *predicted_dev, const uint8_t *expect
  * trial's automata are bit-identical to a solo tm_train_classifier run with
  * the same options.  The tile shape and workgroup order are chosen by timing
  * the candidates on the first epoch (the autotuner); the choice is cached per
- * (row words, trial count, workgroup count).
+ * (row words, state bits, row stride, workgroup count rounded down to a power
+ * of two), so later batches of similar size reuse it.
  */
 typedef struct tm_batch tm_batch;
 
