@@ -130,3 +130,25 @@ def test_state_bits_limits():
     tm.classifier(dict(base, features=16384, state=8))
     with pytest.raises(tm.TsetlinError, match="largest train tile for 9 state bits"):
         tm.classifier(dict(base, features=16384, state=9))
+
+
+def test_batch_tuning_is_per_state_bits():
+    """Two batches with the same row width, trial count and workgroup count,
+    at 8 and then 16 state bits: the second must not reuse a tile tuned for
+    the first (the widest 8-state-bit tile holds 16 words per lane, more than
+    16 state bits fit), and both equal their solo machines."""
+    rng = np.random.default_rng(21)
+    rows = rows_for(4096, 48, rng)
+    y = rng.integers(0, 2, 48).astype(np.int64)
+    trials = [dict(clauses=8, target=4, seed=s, clause_tolerance=5, clause_maximum=6, specificity=400.0)
+              for s in (1, 2)]
+    for state in (8, 16):
+        base = dict(classes=2, features=4096, state=state)
+        b = TR.Batch(base, trials)
+        b.train(48, rows, y, 2)
+        tune = b.tuning()
+        assert tune["words_per_lane"] * state <= 128, (state, tune)
+        for i, t in enumerate(trials):
+            ref = O.Machine(O.params("classifier", **dict(base, **t)))
+            ref.train_replay(rows, labels=y, epochs=2)
+            assert_same_state(b.machine(i), ref, label="state %d trial %d" % (state, i))
