@@ -120,6 +120,18 @@ int tm_predict_encoder(tm_machine *m, const uint8_t *problems, uint32_t n, uint8
 int tm_predict_classifier_dev(tm_machine *m, const uint8_t *problems_dev, uint32_t n, int64_t *out_dev);
 int tm_predict_encoder_dev(tm_machine *m, const uint8_t *problems_dev, uint32_t n, uint8_t *out_dev);
 
+/* tm_predict_classifier_dev without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * class sums, and the rows when input_chunks is not a multiple of 4 or the
+ * rows are not 16-byte aligned (then repacked into whole words), live in a
+ * library-owned device array of 32 MiB per GPU, held until the process ends:
+ * TM_E_SIZE when n * classes * 8 bytes (plus n * ceil(input_chunks / 4) * 4
+ * for repacked rows) exceed it.  Every call on one GPU shares that array, so
+ * calls that overlap on one GPU race.  Same results as
+ * tm_predict_classifier_dev. */
+int tm_predict_classifier_dev_noalloc(tm_machine *m, const uint8_t *problems_dev, uint32_t n, int64_t *out_dev);
+
 /* ---------------------------------------------------------------------------
  * Individualized machines: class h (hidden bit h of an encoder) has a feature
  * set of its own, k_h = feat_offsets[h + 1] - feat_offsets[h] features, and

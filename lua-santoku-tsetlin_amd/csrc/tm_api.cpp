@@ -239,12 +239,25 @@ int size_check(uint64_t classes, uint64_t clauses, uint64_t input_chunks, uint64
   return TM_OK;
 }
 
+// The predict-side views of the include plane (transposed plane, literal
+// counts) are allocated with the machine, so predict itself need not allocate
+// them.
+int alloc_predict_views(tm_machine *m)
+{
+  int rc;
+  if ((rc = m->actT.ensure(m->action_words() * 4))) return rc;
+  if ((rc = m->lit.ensure(m->nclauses() * 4))) return rc;
+  m->predict_ready = false;
+  return TM_OK;
+}
+
 int alloc_device(tm_machine *m)
 {
   int rc;
   const uint64_t aw = m->action_words();
   if ((rc = m->actions.ensure(aw * 4))) return rc;
   if ((rc = m->counts.ensure(aw * (m->state_bits - 1) * 4))) return rc;
+  if ((rc = alloc_predict_views(m))) return rc;
   HIP_TRY(tm_launch_zero(m->actions.p, aw * 4, m->stream));
   m->has_state = true;
   return TM_OK;
@@ -1167,8 +1180,7 @@ int tm_restrict(tm_machine *m, const int64_t *selected, uint32_t n_selected)
     m->counts = nc;
   }
   m->class_chunks = (m->classes + 7) / 8;
-  m->predict_ready = false;
-  return TM_OK;
+  return alloc_predict_views(m);
 }
 
 int tm_get_feature_sizes(const tm_machine *m, uint64_t *out, uint32_t cap)
@@ -1271,7 +1283,7 @@ int tm_load(const uint8_t *data, size_t len, tm_machine **out)
     m->wd = (m->input_chunks + 3) / 4;
   }
   if ((rc = init_runtime(m))) { tm_destroy(m); return rc; }
-  if ((rc = m->actions.ensure(m->action_words() * 4))) { tm_destroy(m); return rc; }
+  if ((rc = m->actions.ensure(m->action_words() * 4)) || (rc = alloc_predict_views(m))) { tm_destroy(m); return rc; }
   if ((rc = actions_from_host(m, acts.data()))) { tm_destroy(m); return rc; }
   m->has_state = false;   // capi.c:1696: state = NULL after load
   m->trained = true;
@@ -1456,6 +1468,104 @@ int tm_last_queue_timing(const tm_machine *m, double *queue_ms)
   if (!m) return fail(TM_E_ARG, "machine is NULL");
   if (queue_ms) *queue_ms = m->queue_ms;
   return TM_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Predict without allocation.  The class sums (and the rows, when they must be
+// repacked into whole words) go to `scratch`, a file-scope device array of
+// kNoAllocScratch bytes that every GPU holds for the life of the process.  Its
+// address on a device comes from hipGetSymbolAddress, taken once per device
+// with that device current; host code uses only that address, never the
+// array's name.  Every call on a GPU shares that GPU's array: calls that
+// overlap on one GPU race.
+
+constexpr size_t kNoAllocScratch = (size_t) 32 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratch];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, uint8_t *> g_scratch_ptr;
+
+// The scratch array's address on the current device (looked up on the first
+// call on that device).
+int scratch_on_device(int device, uint8_t **out)
+{
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) { *out = it->second; return TM_OK; }
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(TM_E_DEVICE, "predict (no allocation): hipGetSymbolAddress of the scratch array failed: %s",
+                hipGetErrorString(e));
+  g_scratch_ptr[device] = (uint8_t *) p;
+  *out = (uint8_t *) p;
+  return TM_OK;
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+} // namespace
+
+extern "C" {
+
+int tm_predict_classifier_dev_noalloc(tm_machine *m, const uint8_t *problems_dev, uint32_t n, int64_t *out_dev)
+{
+  int rc;
+  if ((rc = check_handle(m))) return rc;
+  if (m->type != TM_CLASSIFIER) return fail(TM_E_ARG, "predict: machine is not a classifier");
+  if (m->individualized)
+    return fail(TM_E_ARG, "predict: individualized machine: predict with dim_offsets (the _ind entry points)");
+  if (n == 0) return TM_OK;
+  if (!problems_dev || !out_dev) return fail(TM_E_ARG, "predict: problems and out must not be NULL");
+  const size_t B = m->input_chunks;
+  const bool in_place = (B % 4 == 0) && ((uintptr_t) problems_dev % 16 == 0);
+  const size_t sums_bytes = align_up((size_t) n * m->classes * 8, 256);
+  const size_t rows_bytes = in_place ? 0 : (size_t) n * m->wd * 4;
+  if (sums_bytes + rows_bytes > kNoAllocScratch)
+    return fail(TM_E_SIZE, "predict (no allocation): %u rows need %zu bytes of scratch, the limit is %zu "
+                "(class sums %zu bytes%s)", n, sums_bytes + rows_bytes, kNoAllocScratch, sums_bytes,
+                in_place ? "" : " and rows repacked into whole words");
+  if (!m->actT.p || m->actT.cap < m->action_words() * 4 || !m->lit.p || m->lit.cap < m->nclauses() * 4)
+    return fail(TM_E_STATE, "predict (no allocation): the machine's predict views are not allocated");
+  if ((rc = use_device(m))) return rc;
+  uint8_t *base;
+  if ((rc = scratch_on_device(m->device, &base))) return rc;
+  if (!m->predict_ready) {   // the buffers exist: this only launches the transposes
+    HIP_TRY(tm_launch_prepare_predict(m->actions.as<uint32_t>(), m->actT.as<uint32_t>(), m->lit.as<uint32_t>(),
+                                      (uint32_t) m->nclauses(), m->wd, m->last_eval_mask(), m->stream));
+    m->predict_ready = true;
+  }
+  int64_t *sums = (int64_t *) base;
+  const uint32_t *words = (const uint32_t *) problems_dev;
+  if (!in_place) {
+    uint32_t *w = (uint32_t *)(base + sums_bytes);
+    HIP_TRY(tm_launch_repack(problems_dev, n, (uint32_t) B, m->wd, w, m->stream));
+    words = w;
+  }
+  // a fill kernel, not a memset: the sums start at zero on every call
+  HIP_TRY(tm_launch_zero(sums, (size_t) n * m->classes * 8, m->stream));
+  tm_predict_launch L;
+  memset(&L, 0, sizeof L);
+  L.actT = m->actT.as<uint32_t>();
+  L.lit = m->lit.as<uint32_t>();
+  L.rows = words;
+  L.sums = sums;
+  L.n = n;
+  L.wd = m->wd;
+  L.clauses_total = (uint32_t) m->nclauses();
+  L.clauses = m->clauses;
+  L.classes = m->classes;
+  L.tolerance = m->clause_tolerance;
+  L.last_eval = m->last_eval_mask();
+  HIP_TRY(hipEventRecord(m->ev0, m->stream));
+  HIP_TRY(tm_launch_predict(L, m->stream));
+  HIP_TRY(hipEventRecord(m->ev1, m->stream));
+  HIP_TRY(tm_launch_argmax(sums, n, m->classes, out_dev, m->stream));
+  return predict_finish_timing(m);
 }
 
 } // extern "C"

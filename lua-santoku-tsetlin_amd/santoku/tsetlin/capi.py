@@ -21,7 +21,8 @@ ERRORS = {
 EXPORTS = [
     "tm_options_default", "tm_create_classifier", "tm_create_encoder", "tm_train_classifier",
     "tm_train_encoder", "tm_train_classifier_dev", "tm_train_encoder_dev", "tm_predict_classifier",
-    "tm_predict_encoder", "tm_predict_classifier_dev", "tm_predict_encoder_dev", "tm_persist",
+    "tm_predict_encoder", "tm_predict_classifier_dev", "tm_predict_encoder_dev",
+    "tm_predict_classifier_dev_noalloc", "tm_persist",
     "tm_persist_file", "tm_load", "tm_load_file", "tm_checkpoint", "tm_restore", "tm_checkpoint_dev",
     "tm_restore_dev", "tm_counts", "tm_reconfigure",
     "tm_type", "tm_destroy", "tm_get_info", "tm_last_error", "tm_last_timing", "tm_last_train_stats",
@@ -93,6 +94,7 @@ def load_library(path=LIB_PATH):
         "tm_predict_encoder": (ctypes.c_int, [vp, u8p, u32, u8p]),
         "tm_predict_classifier_dev": (ctypes.c_int, [vp, u8p, u32, i64p]),
         "tm_predict_encoder_dev": (ctypes.c_int, [vp, u8p, u32, u8p]),
+        "tm_predict_classifier_dev_noalloc": (ctypes.c_int, [vp, u8p, u32, i64p]),
         "tm_persist": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, sz_p]),
         "tm_persist_file": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "tm_load": (ctypes.c_int, [u8p, ctypes.c_size_t, m_pp]),
