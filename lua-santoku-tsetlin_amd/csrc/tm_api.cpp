@@ -668,11 +668,12 @@ int build_queue(tm_machine *m, tm_train_launch &L, const uint32_t ek[4], uint32_
   uint64_t total = 0;
   HIP_TRY(hipMemcpyAsync(&total, Q.offsets + (items - 1), 8, hipMemcpyDeviceToHost, m->stream));
   HIP_TRY(hipStreamSynchronize(m->stream));
-  if (total * 20 > kQueueMaxBytes) return TM_OK;
+  // an entry: its id and one 32-bit draw per clause
+  if (total * 36 > kQueueMaxBytes) return TM_OK;
   if (total > m->q_cap || !m->q_id.p || !m->q_draw.p) {
     // some headroom: the count moves a little from epoch to epoch
     const uint64_t cap = std::max<uint64_t>(64, total + total / 8);
-    if ((rc = m->q_id.ensure(cap * 4)) || (rc = m->q_draw.ensure(cap * 16))) {
+    if ((rc = m->q_id.ensure(cap * 4)) || (rc = m->q_draw.ensure(cap * 32))) {
       m->q_id.release();
       m->q_draw.release();
       m->q_cap = 0;

@@ -58,7 +58,7 @@ struct tm_train_launch {
   // [qoff[ch * qseg], qoff[(ch + 1) * qseg]) in order.
   const uint64_t *qoff;
   const uint32_t *qid;            // sample | target << 31
-  const uint32_t *qdraw;          // [4][qcap] step-draw words (Philox TAG_STEP output)
+  const uint32_t *qdraw;          // [8][qcap] step draws, clause j's in plane j (Philox TAG_STEP, TAG_STEP + 1)
   uint64_t qcap;
   uint32_t qseg;                  // 1024-position segments per chunk
 };

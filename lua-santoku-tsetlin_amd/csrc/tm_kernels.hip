@@ -361,11 +361,17 @@ __global__ __launch_bounds__(256) void queue_count_kernel(tm_queue_launch Q)
 __device__ __forceinline__ void queue_emit(const tm_queue_launch &Q, uint32_t e, uint64_t idx, uint32_t ch)
 {
   Q.qid[idx] = e;
+  // the eight clauses' draws: words 0-3 and 4-7 of the entry, plane-major
   const tm_u4 d = tm_philox(e & 0x7FFFFFFFu, ch, Q.epoch, TM_TAG_STEP, Q.seed_lo, Q.seed_hi);
   Q.qdraw[idx] = d.x;
   Q.qdraw[Q.qcap + idx] = d.y;
   Q.qdraw[2 * Q.qcap + idx] = d.z;
   Q.qdraw[3 * Q.qcap + idx] = d.w;
+  const tm_u4 d2 = tm_philox(e & 0x7FFFFFFFu, ch, Q.epoch, TM_TAG_STEP + 1, Q.seed_lo, Q.seed_hi);
+  Q.qdraw[4 * Q.qcap + idx] = d2.x;
+  Q.qdraw[5 * Q.qcap + idx] = d2.y;
+  Q.qdraw[6 * Q.qcap + idx] = d2.z;
+  Q.qdraw[7 * Q.qcap + idx] = d2.w;
 }
 
 __device__ __forceinline__ void wave_lds_sync()

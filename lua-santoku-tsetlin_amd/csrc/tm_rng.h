@@ -11,9 +11,11 @@
 //   sample order        = positions sorted by (Philox(key, (i, epoch, 0, TAG_PERM))[0], i)
 //   skip draw (s, ch)   = lowbias32(lowbias32(s ^ ek0) ^ (ch*GOLDEN + ek1))
 //                         (negative sampling, capi.c:215)
-//   step draws (s, ch)  = Philox(key, (s, ch, epoch, TAG_STEP)): 8 x 16 bits,
-//                         clause j of the chunk uses bits [16*(j&1), +16) of word j>>1
-//                         (the chance(p) pair draws, capi.c:232-239)
+//   step draw (s, ch, j) = word j&3 of Philox(key, (s, ch, epoch, TAG_STEP + (j>>2))):
+//                         one 32-bit draw per clause j of the chunk; clause j
+//                         fires iff mulhi(draw, 2T) < T -+ v, i.e. with
+//                         probability (T -+ v) / 2T to within 2^-32
+//                         (the chance(p) pair draws, capi.c:221-239)
 //   literal key (s, cl) = lowbias32(lowbias32(s ^ ek2) ^ (cl*GOLDEN + ek3))
 //   literal draw r      = (lowbias32(kev + r*GOLDEN) * input_bits) >> 32
 //                         (the "random % input_bits" of capi.c:188)
@@ -31,7 +33,7 @@
 
 #define TM_TAG_EPOCH 0x45504f43u
 #define TM_TAG_PERM 0x5045524du
-#define TM_TAG_STEP 0x53544550u
+#define TM_TAG_STEP 0x53544550u     // clauses 0-3 of a chunk; TM_TAG_STEP + 1: clauses 4-7
 #define TM_GOLDEN 0x9E3779B9u
 
 struct tm_u4 { uint32_t x, y, z, w; };

@@ -368,23 +368,27 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
     // entries, loaded one block ahead.
     uint32_t b0 = 0, bs = 0;
     uint64_t want = 0, tmask = 0;
-    tm_u4 bd = { 0, 0, 0, 0 };
-    uint32_t bsel = 0;                 // full-wave group: this clause's draw word of bd
+    tm_u4 bd = { 0, 0, 0, 0 }, bd2 = { 0, 0, 0, 0 };   // smaller groups: all 8 clauses' draws
+    uint32_t bsel = 0;                 // full-wave group: this clause's draw word
     uint64_t qb = 0, qe = 0;           // queue entries not yet loaded
     uint32_t qi_n = 0, qd_n = 0;       // prefetched block
-    tm_u4 qd4_n = { 0, 0, 0, 0 };
+    tm_u4 qd4_n = { 0, 0, 0, 0 }, qd4b_n = { 0, 0, 0, 0 };
     auto qload = [&]() {              // called with qb < qe
       // lanes past the chunk's last entry re-read it: the load stays
       // unconditional, so it can land in place without a wait
       const uint64_t idx = min(qb + lane, qe - 1);
       qi_n = L.qid[idx];
       if constexpr (G == 64) {
-        qd_n = L.qdraw[(uint64_t)(j >> 1) * L.qcap + idx];
+        qd_n = L.qdraw[(uint64_t) j * L.qcap + idx];
       } else {
         qd4_n.x = L.qdraw[idx];
         qd4_n.y = L.qdraw[L.qcap + idx];
         qd4_n.z = L.qdraw[2 * L.qcap + idx];
         qd4_n.w = L.qdraw[3 * L.qcap + idx];
+        qd4b_n.x = L.qdraw[4 * L.qcap + idx];
+        qd4b_n.y = L.qdraw[5 * L.qcap + idx];
+        qd4b_n.z = L.qdraw[6 * L.qcap + idx];
+        qd4b_n.w = L.qdraw[7 * L.qcap + idx];
       }
     };
     if constexpr (QUEUE) {
@@ -404,6 +408,7 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
           asm volatile("" : "+v"(bsel));
         } else {
           bd = qd4_n;
+          bd2 = qd4b_n;
         }
         qb += 64;
         if (qb < qe) qload();
@@ -427,8 +432,14 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
           if (want) {
             tmask = __ballot(tgt);
             bs = s;
-            bd = tm_philox(s, chr, epoch, TM_TAG_STEP, P.seed_lo, P.seed_hi);
-            if constexpr (G == 64) bsel = (j >> 1) == 0 ? bd.x : (j >> 1) == 1 ? bd.y : (j >> 1) == 2 ? bd.z : bd.w;
+            if constexpr (G == 64) {
+              // a full-wave group draws only its own clause's Philox block
+              bd = tm_philox(s, chr, epoch, TM_TAG_STEP + (j >> 2), P.seed_lo, P.seed_hi);
+              bsel = (j & 3) == 0 ? bd.x : (j & 3) == 1 ? bd.y : (j & 3) == 2 ? bd.z : bd.w;
+            } else {
+              bd = tm_philox(s, chr, epoch, TM_TAG_STEP, P.seed_lo, P.seed_hi);
+              bd2 = tm_philox(s, chr, epoch, TM_TAG_STEP + 1, P.seed_lo, P.seed_hi);
+            }
             return true;
           }
         }
@@ -449,15 +460,18 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
         nq_s = __builtin_amdgcn_readlane(bs, q);
         nq_t = (tmask >> q) & 1;
       }
-      uint32_t dsel;
       if constexpr (G == 64) {
-        dsel = __builtin_amdgcn_readlane(bsel, q);
+        nq_d = __builtin_amdgcn_readlane(bsel, q);
       } else {
+        // lane q's eight draws, then this lane's clause's (j differs per lane)
         const uint32_t d0 = __builtin_amdgcn_readlane(bd.x, q), d1 = __builtin_amdgcn_readlane(bd.y, q);
         const uint32_t d2 = __builtin_amdgcn_readlane(bd.z, q), d3 = __builtin_amdgcn_readlane(bd.w, q);
-        dsel = (j >> 1) == 0 ? d0 : (j >> 1) == 1 ? d1 : (j >> 1) == 2 ? d2 : d3;
+        const uint32_t d4 = __builtin_amdgcn_readlane(bd2.x, q), d5 = __builtin_amdgcn_readlane(bd2.y, q);
+        const uint32_t d6 = __builtin_amdgcn_readlane(bd2.z, q), d7 = __builtin_amdgcn_readlane(bd2.w, q);
+        const uint32_t lo = (j & 3) == 0 ? d0 : (j & 3) == 1 ? d1 : (j & 3) == 2 ? d2 : d3;
+        const uint32_t hi = (j & 3) == 0 ? d4 : (j & 3) == 1 ? d5 : (j & 3) == 2 ? d6 : d7;
+        nq_d = (j >> 2) ? hi : lo;
       }
-      nq_d = (dsel >> (16 * (j & 1))) & 0xFFFFu;
     };
     uint32_t xn[WPL];
     // FULL rows are G * WPL words apart (a power of two): a shift, not a multiply
@@ -470,7 +484,7 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
     take();
     fetch(nq_s);
     for (;;) {
-      const uint32_t sq = nq_s, u16 = nq_d;
+      const uint32_t sq = nq_s, u32 = nq_d;
       const bool tq = nq_t;
       // x is defined here, by an empty asm on the copy: without it the
       // compiler renames x onto the prefetch registers across its loop
@@ -522,12 +536,11 @@ __device__ __forceinline__ void train_body(const tm_train_launch &L)
       vr ^= 32;
       if (v > (int32_t) T) v = (int32_t) T;
       if (v < -(int32_t) T) v = -(int32_t) T;
-      // fire iff u16 * 2T < (T -+ v) * 2^16; for integers x < num * 2^16
-      // exactly when x >> 16 < num.  x >> 16 of the 48-bit product fits in
-      // 32 bits (T < 2^31), taken from its two 32-bit halves.
+      // chance((T -+ v) / 2T): fire iff u32 * 2T < (T -+ v) * 2^32, which for
+      // integers holds exactly when the product's high word is below T -+ v
+      // (2T < 2^32: the host keeps T below 2^30)
       const uint32_t num = (uint32_t)(tq ? (int32_t) T - v : (int32_t) T + v);
-      const uint32_t prod_lo = u16 * T2, prod_hi = __umulhi(u16, T2);
-      const bool fire = ((prod_hi << 16) | (prod_lo >> 16)) < num;
+      const bool fire = __umulhi(u32, T2) < num;
       n_steps++;
       if (fire) {
         const bool type1 = tq ^ (j & 1);

@@ -183,12 +183,16 @@ uint32_t orc_skip_draw (const uint32_t ek[4], uint32_t sample, uint32_t chunk)
   return orc_lowbias32(orc_lowbias32(sample ^ ek[0]) ^ (chunk * ORC_GOLDEN + ek[1]));
 }
 
-/* Eight 16-bit feedback draws for (sample, chunk) in an epoch (capi.c:232-239). */
-void orc_step_draws (uint64_t seed, uint32_t epoch, uint32_t sample, uint32_t chunk, uint32_t d[4])
+/* The eight 32-bit feedback draws for (sample, chunk) in an epoch, d[j] for
+ * clause j of the chunk (capi.c:221-239): clauses 0-3 from one Philox block,
+ * 4-7 from the next counter tag. */
+void orc_step_draws (uint64_t seed, uint32_t epoch, uint32_t sample, uint32_t chunk, uint32_t d[8])
 {
   uint32_t key[2], ctr[4] = { sample, chunk, epoch, ORC_TAG_STEP };
   seed_key(seed, key);
   orc_philox(ctr, key, d);
+  ctr[3] = ORC_TAG_STEP + 1;
+  orc_philox(ctr, key, d + 4);
 }
 
 /* Key of the random-literal draws of a Type I / output 0 event on a clause. */
@@ -521,12 +525,12 @@ static void replay_step (const orc_params *p, uint8_t *counts, uint8_t *actions,
   if (v < -T) v = -T;
   uint64_t num = (uint64_t)(tgt ? T - v : T + v);                   /* capi.c:221-225 */
   uint64_t den = 2 * (uint64_t) T;
-  uint32_t d[4];
+  uint32_t d[8];
   orc_step_draws(p->seed, epoch, sample, chunk_r, d);
   st[0]++;
   for (uint32_t j = 0; j < 8; j++) {                                /* capi.c:226-242 */
-    uint64_t u = (d[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
-    if (!(u * den < (num << 16))) continue;
+    /* chance(num / den): u / 2^32 < num / den, exactly in integers */
+    if (!((uint64_t) d[j] * den < (num << 32))) continue;
     bool type1 = tgt ? ((j & 1) == 0) : ((j & 1) == 1);
     uint64_t clause = (uint64_t) chunk * 8 + j;
     bool output = vot[j] > 0;                                       /* capi.c:176 */

@@ -86,3 +86,23 @@ def test_device_buffer_checks():
         tm._device_buffer(FakeTensor("torch.uint8", 99), np.uint8, 100, "rows")
     with pytest.raises(tm.TsetlinError, match="holds 72 bytes"):
         tm._device_buffer(FakeTensor("torch.int64", 9, itemsize=8), np.int64, 80, "labels")
+
+
+def test_width_limits_refused_before_any_device_call():
+    """Tolerances from 2^25 and targets from 2^30 exceed the kernels' 32-bit
+    vote and fire arithmetic: refused with TM_E_UNSUPPORTED while the options
+    are checked, before the device is touched (without a GPU, a machine that
+    passes these checks fails later with TM_E_DEVICE)."""
+    import santoku.tsetlin as tm
+    from conftest import gpu_available
+    base = dict(classes=2, features=8, clauses=8, clause_maximum=3, specificity=2.0)
+    with pytest.raises(tm.TsetlinError, match="TM_E_UNSUPPORTED.*clause_tolerance 33554432"):
+        tm.classifier(dict(base, clause_tolerance=1 << 25))
+    with pytest.raises(tm.TsetlinError, match="TM_E_UNSUPPORTED.*target"):
+        tm.classifier(dict(base, clause_tolerance=4, target=float(1 << 30)))
+    # the default target sqrt(clauses / 2) * tolerance can reach the limit too
+    with pytest.raises(tm.TsetlinError, match="TM_E_UNSUPPORTED.*target"):
+        tm.classifier(dict(base, clauses=1 << 20, clause_tolerance=(1 << 25) - 1))
+    if not gpu_available():
+        with pytest.raises(tm.TsetlinError, match="TM_E_DEVICE"):
+            tm.classifier(dict(base, clause_tolerance=(1 << 25) - 1, target=float((1 << 30) - 1)))

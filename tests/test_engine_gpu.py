@@ -424,3 +424,28 @@ def test_zero_iterations_runs_setup():
     eng.restore(np.full(32, 0xFF, np.uint8))
     eng.train(samples=0, problems=np.zeros(0, np.uint8), solutions=np.zeros(0, np.int64), iterations=0)
     assert (eng.checkpoint() == 0).all()
+
+
+@pytest.mark.parametrize("kind,target", [("classifier", 40000), ("classifier", 3_000_000),
+                                         ("classifier", (1 << 30) - 1), ("encoder", 123_457)])
+def test_fire_test_at_large_targets(kind, target):
+    """Targets past 2^15, up to the 2^30 - 1 limit: every clause's feedback
+    draw is 32 bits and the fire test mulhi(u, 2T) < T -+ v stays exact, on the
+    four-workgroup queue tile (classifier) and the scan tile (encoder)."""
+    rng = np.random.default_rng(target % 1000)
+    n = 200
+    rows = rows_for(1024, n, rng, 0.3)
+    o = dict(features=1024, classes=3, clauses=16, clause_tolerance=64, clause_maximum=64, specificity=16.0,
+             target=target, seed=29)
+    eng, orc = pair(kind, **o)
+    if kind == "classifier":
+        labels = rng.integers(0, 3, n).astype(np.int64)
+        eng.train(samples=n, problems=rows, solutions=labels, iterations=2)
+        orc.train_replay(rows, labels=labels, epochs=2)
+    else:
+        codes = rng.integers(0, 8, (n, 1), dtype=np.uint8)
+        eng.train(samples=n, sentences=rows, codes=codes, iterations=2)
+        orc.train_replay(rows, codes=codes, epochs=2)
+    assert eng.info()["target"] == target
+    assert orc.stats[1] > 0 and orc.stats[3] > 0
+    assert_same_state(eng, orc, label="target %d" % target)

@@ -11,6 +11,7 @@ checked against something that does not share its formulation:
     same RNG stream.
 """
 import ctypes
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -291,11 +292,12 @@ def py_train(p, rows, labels, codes, epochs):
                 cv = int(sum(votes[j] if j % 2 == 0 else -votes[j] for j in range(8)))
                 cv = max(-T, min(T, cv))
                 num = T - cv if tgt else T + cv
-                d = np.zeros(4, np.uint32)
+                d = np.zeros(8, np.uint32)
                 L.orc_step_draws(p.seed, e, s, ch, O.ptr(d, O._U32P))
                 for j in range(8):
-                    u = (int(d[j >> 1]) >> (16 * (j & 1))) & 0xFFFF
-                    if not (u * 2 * T < (num << 16)):
+                    # chance(p), p = num / 2T (capi.c:221-225): a uniform
+                    # u / 2^32 below p, compared as exact fractions
+                    if not Fraction(int(d[j]), 1 << 32) < Fraction(num, 2 * T):
                         continue
                     type1 = (j % 2 == 0) if tgt else (j % 2 == 1)
                     cl = cl0 + j
