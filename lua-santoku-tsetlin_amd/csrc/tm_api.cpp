@@ -6,6 +6,7 @@
 // memory; the kernels are in tm_kernels.hip.
 #include "tsetlin_amd.h"
 #include "tm_internal.h"
+#include "tm_layout.h"
 #include "tm_rng.h"
 
 #include <hip/hip_runtime.h>
@@ -66,29 +67,8 @@ struct dbuf {
   template <typename T> T *as() const { return (T *) p; }
 };
 
-// Literal bits < input_bits inside the last word of a row of wd words.
-uint32_t eval_mask(uint32_t wd, uint32_t input_bits)
-{
-  uint32_t bits = input_bits - 32 * (wd - 1);
-  return bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1);
-}
-
-// Bits of the last word whose automata evolve.  Bytes < input_chunks are live.
-// The last byte keeps only its tail bits when the reference's masked byte path
-// handles it (input_chunks % 16 != 0, automata.h:97,106); inside the unmasked
-// 128-bit bulk region all 8 bits of the byte evolve (automata.h:164-194).
-uint32_t live_mask(uint32_t wd, uint32_t input_chunks, uint8_t tail_mask)
-{
-  uint32_t m = 0;
-  for (uint32_t b = 0; b < 4; b++) {
-    uint32_t byte = 4 * (wd - 1) + b;
-    if (byte >= input_chunks) break;
-    uint8_t bm = 0xFF;
-    if (byte == input_chunks - 1 && (input_chunks % 16) != 0) bm = tail_mask;
-    m |= (uint32_t) bm << (8 * b);
-  }
-  return m;
-}
+using tm_layout::eval_mask;
+using tm_layout::live_mask;
 
 // A trial whose rows are wd words apart, wd words long, with no bit of the
 // last word masked: the train kernel's variant for such rows keeps no masks.
@@ -107,13 +87,7 @@ int tm_set_error(int code, const char *msg)
 
 // One class of an individualized machine: a one-class machine over its own
 // feature set (capi.c:576-604), stored after the classes before it.
-struct tm_ind_class {
-  uint64_t features = 0;                  // k_h
-  uint32_t input_bits = 0, input_chunks = 0, wd = 0;
-  uint8_t tail_mask = 0xFF;
-  uint64_t aoff = 0, coff = 0;            // u32-word offsets of its actions and counts
-  uint64_t hoff = 0;                      // byte offset of its actions in the persisted layout
-};
+using tm_ind_class = tm_layout::ind_class;
 
 struct tm_machine {
   int type = TM_CLASSIFIER;
@@ -171,24 +145,18 @@ int use_device(tm_machine *m)
   return TM_OK;
 }
 
-// Widths of the kernels' integer arithmetic.  A vote is at most the
-// tolerance and a predict tile sums 64 of them in 32 bits; the train kernel's
-// fire test computes T +- v and 2T in 32 bits.  The reference computes in
-// long int and double (capi.c:148-161, :217-225) and has no such limits; class
-// sums here are 64-bit as there, so only these two remain.
-const uint32_t kMaxTolerance = (1u << 25) - 1;
-const double kMaxTarget = (double) ((1u << 30) - 1);
+// The kernels' integer widths (tm_layout.h).  The reference computes in long
+// int and double (capi.c:148-161, :217-225) and has no such limits; class
+// sums here are 64-bit as there, so only the tolerance and target remain.
+using tm_layout::kMaxTolerance;
+using tm_layout::kMaxTarget;
 
 // capi.c:473-493: derived sizes and defaults.  The target keeps the reference's
 // float fmaxf and truncating conversion to unsigned int (capi.c:480-483).
 // Targets above kMaxTarget are refused before the conversion.
 int derive_target(double targetf, uint32_t clauses, uint32_t tolerance, const char *what, uint32_t *out)
 {
-  double t;
-  if (targetf < 0)
-    t = sqrt((double) clauses / 2.0) * (double) tolerance;
-  else
-    t = ceil(targetf >= 1 ? targetf : fmaxf(1.0f, (float)((double) clauses * targetf)));
+  const double t = tm_layout::target_value(targetf, clauses, tolerance);
   if (!(t <= kMaxTarget))
     return fail(TM_E_UNSUPPORTED, "%s: target %.17g exceeds %.0f, the train kernel's 32-bit feedback arithmetic",
                 what, t, kMaxTarget);
@@ -211,6 +179,9 @@ int validate_opts(const tm_options *o, const char *what, bool ind = false)
   if (!o) return fail(TM_E_ARG, "%s: options are NULL", what);
   if (!o->classes) return fail(TM_E_ARG, "%s: classes must be greater than 1", what);
   if (!o->clauses) return fail(TM_E_ARG, "%s: clauses must be greater than 0", what);
+  // rounding up to whole bytes (capi.c:477) must not wrap 32 bits
+  if (o->classes > 0x7FFFFFFFu || o->clauses > 0x7FFFFFF8u)
+    return fail(TM_E_ARG, "%s: classes or clauses too large for the 32-bit size fields", what);
   if (!o->clause_tolerance) return fail(TM_E_ARG, "%s: clause_tolerance must be greater than 0", what);
   if (!o->clause_maximum) return fail(TM_E_ARG, "%s: clause_maximum must be greater than 0", what);
   if (o->state < 2) return fail(TM_E_ARG, "%s: bits must be greater than 1", what);
@@ -227,9 +198,7 @@ int validate_opts(const tm_options *o, const char *what, bool ind = false)
 // only predicts).
 int size_check(uint64_t classes, uint64_t clauses, uint64_t input_chunks, uint64_t state_bits, bool train = true)
 {
-  uint64_t act = classes * clauses * input_chunks;
-  uint64_t sta = act * (state_bits - 1);
-  if (act > 0xFFFFFFFFull || sta > 0xFFFFFFFFull)
+  if (!tm_layout::state_fits(classes, clauses, input_chunks, state_bits))
     return fail(TM_E_ARG, "machine too large for the persisted 32-bit size fields");
   uint32_t lanes, wpl;
   if (train && tm_train_shape((uint32_t)((input_chunks + 3) / 4), (uint32_t) state_bits, &lanes, &wpl))
@@ -270,31 +239,23 @@ int alloc_device(tm_machine *m)
 // tile's limits (a loaded machine only predicts).
 int ind_layout(tm_machine *m, const char *what, bool train = true)
 {
-  uint64_t aw = 0, hb = 0;
-  uint32_t wmax = 1;
-  for (size_t h = 0; h < m->ind.size(); h++) {
-    tm_ind_class &c = m->ind[h];
-    if (c.features == 0) return fail(TM_E_ARG, "%s: class %zu has an empty feature set", what, h);
-    if (c.features > 0x7FFFFFFFull) return fail(TM_E_ARG, "%s: class %zu has too many features", what, h);
-    c.input_bits = (uint32_t)(2 * c.features);
-    c.input_chunks = (c.input_bits + 7) / 8;
-    const uint32_t tail_bits = c.input_bits & 7;
-    c.tail_mask = tail_bits ? (uint8_t)((1u << tail_bits) - 1) : 0xFF;
-    c.wd = (c.input_chunks + 3) / 4;
-    int rc = size_check(1, m->clauses, c.input_chunks, m->state_bits, train);
-    if (rc) return rc;
-    c.aoff = aw;
-    c.coff = aw * (m->state_bits - 1);
-    c.hoff = hb;
-    aw += (uint64_t) m->clauses * c.wd;
-    hb += (uint64_t) m->clauses * c.input_chunks;
-    wmax = std::max(wmax, c.wd);
+  tm_layout::ind_totals t;
+  const char *why = "";
+  const size_t bad = tm_layout::ind_layout(m->ind.data(), m->ind.size(), m->clauses, m->state_bits, &t, &why);
+  if (bad) {
+    if (bad > m->ind.size()) return fail(TM_E_ARG, "%s: machine too large for the persisted 32-bit size fields", what);
+    if (!strcmp(why, "empty")) return fail(TM_E_ARG, "%s: class %zu has an empty feature set", what, bad - 1);
+    if (!strcmp(why, "width")) return fail(TM_E_ARG, "%s: class %zu has too many features", what, bad - 1);
+    return fail(TM_E_ARG, "%s: class %zu: machine too large for the persisted 32-bit size fields", what, bad - 1);
   }
-  if (hb * (m->state_bits - 1) > 0xFFFFFFFFull)
-    return fail(TM_E_ARG, "%s: machine too large for the persisted 32-bit size fields", what);
-  m->action_chunks = (uint32_t) hb;
-  m->state_chunks = (uint32_t)(hb * (m->state_bits - 1));
-  m->wd = wmax;
+  if (train)
+    for (const tm_ind_class &c : m->ind) {
+      int rc = size_check(1, m->clauses, c.input_chunks, m->state_bits, true);
+      if (rc) return rc;
+    }
+  m->action_chunks = t.action_chunks;
+  m->state_chunks = t.state_chunks;
+  m->wd = t.wd_max;
   return TM_OK;
 }
 
@@ -337,10 +298,11 @@ int create_common(const tm_options *o, int type, tm_machine **out, hipStream_t s
   m->features = o->features;
   m->state_bits = o->state;
   m->include_bits = o->include_bits ? o->include_bits : 1;
-  m->input_bits = 2 * o->features;
-  uint32_t tail_bits = m->input_bits & 7;
-  m->tail_mask = tail_bits ? (uint8_t)((1u << tail_bits) - 1) : 0xFF;
-  m->input_chunks = (m->input_bits + 7) / 8;
+  tm_layout::sizes z;
+  if (!tm_layout::row_sizes(o->features, &z)) { delete m; return fail(TM_E_ARG, "%s: too many features", what); }
+  m->input_bits = z.input_bits;
+  m->tail_mask = z.tail_mask;
+  m->input_chunks = z.input_chunks;
   m->clause_chunks = m->clauses / 8;
   if ((rc = size_check(m->classes, m->clauses, m->input_chunks, m->state_bits))) { delete m; return rc; }
   m->state_chunks = m->classes * m->clauses * (m->state_bits - 1) * m->input_chunks;
@@ -348,7 +310,7 @@ int create_common(const tm_options *o, int type, tm_machine **out, hipStream_t s
   m->specificity = o->specificity;
   m->reusable = o->reusable != 0;
   m->seed = o->seed;
-  m->wd = (m->input_chunks + 3) / 4;
+  m->wd = z.wd;
   if ((rc = init_runtime(m, shared)) || (rc = alloc_device(m))) { tm_destroy(m); return rc; }
   *out = m;
   return TM_OK;
@@ -1229,30 +1191,26 @@ int tm_load(const uint8_t *data, size_t len, tm_machine **out)
   if (!out) return fail(TM_E_ARG, "load: out is NULL");
   *out = nullptr;
   if (!data) return fail(TM_E_ARG, "load: data is NULL");
-  reader r{ data, len };
-  int32_t type;
-  uint32_t h[13];
-  double spec;
-  uint8_t tail;
-  if (!r.take(&type, 4)) return fail(TM_E_FORMAT, "load: truncated data");
-  if (type != TM_CLASSIFIER && type != TM_ENCODER) return fail(TM_E_FORMAT, "unexpected tsetlin machine type in load");
-  if (!r.take(h, sizeof h) || !r.take(&spec, 8) || !r.take(&tail, 1)) return fail(TM_E_FORMAT, "load: truncated header");
+  tm_layout::header h;
+  if (const char *e = tm_layout::read_header(data, len, &h)) return fail(TM_E_FORMAT, "%s", e);
+  if (const char *e = tm_layout::check_common(h)) return fail(TM_E_FORMAT, "%s", e);
+  reader r{ data, len, tm_layout::kHeaderBytes };
   tm_machine *m = new tm_machine();
-  m->type = type;
-  m->classes = h[0]; m->features = h[1]; m->clauses = h[2]; m->clause_tolerance = h[3];
-  m->clause_maximum = h[4]; m->target = h[5]; m->state_bits = h[6]; m->include_bits = h[7] ? h[7] : 1;
-  m->input_bits = h[8]; m->input_chunks = h[9]; m->clause_chunks = h[10]; m->state_chunks = h[11];
-  m->action_chunks = h[12];
-  m->specificity = spec;
-  m->tail_mask = tail;
+  m->type = h.type;
+  m->classes = h.classes; m->features = h.features; m->clauses = h.clauses; m->clause_tolerance = h.clause_tolerance;
+  m->clause_maximum = h.clause_maximum; m->target = h.target; m->state_bits = h.state_bits;
+  m->include_bits = h.include_bits ? h.include_bits : 1;
+  m->input_bits = h.input_bits; m->input_chunks = h.input_chunks; m->clause_chunks = h.clause_chunks;
+  m->state_chunks = h.state_chunks; m->action_chunks = h.action_chunks;
+  m->specificity = h.specificity;
+  m->tail_mask = h.tail_mask;
   m->class_chunks = (m->classes + 7) / 8;
-  m->negative = m->classes ? 1.0 / (double) m->classes : 0;   // not persisted (capi.c:1509-1531)
-  if (!m->classes || !m->clauses || m->clauses % 8 || m->clause_chunks != m->clauses / 8) {
-    delete m;
-    return fail(TM_E_FORMAT, "load: inconsistent header");
-  }
+  m->negative = 1.0 / (double) m->classes;   // not persisted (capi.c:1509-1531)
   int rc;
   if ((rc = check_tolerance(m->clause_tolerance, "load"))) { delete m; return rc; }
+  // the actions and the individualized byte must be there before any buffer
+  // is sized from the header
+  if ((uint64_t) m->action_chunks + 1 > len - r.off) { delete m; return fail(TM_E_FORMAT, "load: truncated actions"); }
   std::vector<uint8_t> acts(m->action_chunks);
   uint8_t ind = 0;
   if (!r.take(acts.data(), acts.size()) || !r.take(&ind, 1)) { delete m; return fail(TM_E_FORMAT, "load: truncated actions"); }
@@ -1274,11 +1232,7 @@ int tm_load(const uint8_t *data, size_t len, tm_machine **out)
     }
     if (m->action_chunks != persisted_actions) { delete m; return fail(TM_E_FORMAT, "load: inconsistent header"); }
   } else {
-    if (m->input_bits == 0 || m->input_chunks != (m->input_bits + 7) / 8 ||
-        (uint64_t) m->action_chunks != (uint64_t) m->classes * m->clauses * m->input_chunks) {
-      delete m;
-      return fail(TM_E_FORMAT, "load: inconsistent header");
-    }
+    if (const char *e = tm_layout::check_plain(h)) { delete m; return fail(TM_E_FORMAT, "%s", e); }
     if ((rc = size_check(m->classes, m->clauses, m->input_chunks, m->state_bits < 2 ? 2 : m->state_bits, false))) { delete m; return rc; }
     m->features = m->input_bits / 2;
     m->wd = (m->input_chunks + 3) / 4;
@@ -1374,6 +1328,7 @@ int tm_reconfigure(tm_machine *m, const tm_options *o)
   if (!m->reusable) return fail(TM_E_STATE, "reconfigure requires reusable=true at creation time");
   if (!o) return fail(TM_E_ARG, "reconfigure: options are NULL");
   if (!o->clauses) return fail(TM_E_ARG, "reconfigure: clauses must be greater than 0");
+  if (o->clauses > 0x7FFFFFF8u) return fail(TM_E_ARG, "reconfigure: clauses too large for the 32-bit size fields");
   if (!o->clause_tolerance) return fail(TM_E_ARG, "reconfigure: clause_tolerance must be greater than 0");
   if (!o->clause_maximum) return fail(TM_E_ARG, "reconfigure: clause_maximum must be greater than 0");
   if (!(o->specificity > 0)) return fail(TM_E_ARG, "reconfigure: specificity must be a positive number");

@@ -13,6 +13,7 @@
 // coalesced, and the complemented literal rows staged in LDS and read as
 // wave-wide broadcasts.
 #include "tm_internal.h"
+#include "tm_layout.h"
 #include "tm_rng.h"
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
@@ -454,14 +455,6 @@ int launch_train_shape(const tm_train_launch &L, hipStream_t st)
   return (int) hipGetLastError();
 }
 
-// instantiated tiles: lanes per clause x words per lane
-const uint32_t kShapes[][2] = { { 8, 1 }, { 8, 2 }, { 16, 1 }, { 16, 2 }, { 32, 1 }, { 32, 2 },
-                                { 64, 1 }, { 64, 2 }, { 64, 4 }, { 64, 8 }, { 64, 16 } };
-
-// A tile is built for a plane count when the chunk state of one lane,
-// (planes + 1) x words per lane registers, fits in 128 VGPRs.
-bool tile_fits(uint32_t wpl, uint32_t state_bits) { return state_bits * wpl <= 128; }
-
 } // namespace
 
 // Train kernels of the other state bits (tm_train_states.hip, one object per
@@ -475,35 +468,22 @@ TM_PLANES_DECL(14) TM_PLANES_DECL(15)
 // ---------------------------------------------------------------------------
 // launchers
 
-// Default tile: the smallest lanes x words that covers the row, one clause
-// group per wave once a row needs 64 lanes.
+// Default tile and the autotuner's candidates (tm_layout.h).
 int tm_train_shape(uint32_t wd, uint32_t state_bits, uint32_t *lanes, uint32_t *wpl)
 {
-  static const uint32_t order[][2] = { { 8, 1 }, { 16, 1 }, { 32, 1 }, { 64, 1 }, { 64, 2 }, { 64, 4 }, { 64, 8 }, { 64, 16 } };
-  if (state_bits < 2 || state_bits > TM_MAX_STATE_BITS) return -1;
-  for (auto &s : order)
-    if (s[0] * s[1] >= wd && tile_fits(s[1], state_bits)) { *lanes = s[0]; *wpl = s[1]; return 0; }
-  return -1;
+  return tm_layout::train_shape(wd, state_bits, lanes, wpl);
 }
 
 int tm_train_shapes(uint32_t wd, uint32_t state_bits, uint32_t (*out)[2], int cap)
 {
-  int n = 0;
-  if (state_bits < 2 || state_bits > TM_MAX_STATE_BITS) return 0;
-  for (auto &s : kShapes)
-    if (s[0] * s[1] >= wd && s[0] * s[1] < 4 * wd + 8 && tile_fits(s[1], state_bits) && n < cap) {
-      out[n][0] = s[0];
-      out[n][1] = s[1];
-      n++;
-    }
-  return n;
+  return tm_layout::train_shapes(wd, state_bits, out, cap);
 }
 
 int tm_launch_train(const tm_train_launch &L, hipStream_t st)
 {
   uint32_t G = L.lanes, W = L.wpl;
   if (!G && tm_train_shape(L.wd, L.state_bits, &G, &W)) return (int) hipErrorInvalidValue;
-  if (L.state_bits < 2 || L.state_bits > TM_MAX_STATE_BITS || !tile_fits(W, L.state_bits))
+  if (L.state_bits < 2 || L.state_bits > TM_MAX_STATE_BITS || !tm_layout::tile_fits(W, L.state_bits))
     return (int) hipErrorInvalidValue;
   // L.wd is the widest trial's row; the host checks each trial's stride
   if (L.row_stride % W != 0 || L.row_stride < L.wd || (uint64_t) G * W < L.wd) return (int) hipErrorInvalidValue;

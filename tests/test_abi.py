@@ -106,3 +106,13 @@ def test_width_limits_refused_before_any_device_call():
     if not gpu_available():
         with pytest.raises(tm.TsetlinError, match="TM_E_DEVICE"):
             tm.classifier(dict(base, clause_tolerance=(1 << 25) - 1, target=float((1 << 30) - 1)))
+
+
+def test_sizes_that_would_wrap_are_refused():
+    """Clause and class counts whose rounding to whole bytes would wrap 32 bits
+    are refused as arguments, before the device is touched."""
+    import santoku.tsetlin as tm
+    base = dict(classes=2, features=8, clause_tolerance=3, clause_maximum=3, specificity=2.0)
+    for o in (dict(base, clauses=0xFFFFFFFF), dict(base, clauses=8, classes=0xFFFFFFFE)):
+        with pytest.raises(tm.TsetlinError, match="TM_E_ARG.*too large"):
+            tm.classifier(o)
