@@ -141,7 +141,8 @@ def test_train_workload1_bitexact():
 
 @pytest.mark.parametrize("features,kind,state", [(6, "classifier", 8), (40, "encoder", 8), (300, "classifier", 8),
                                                  (24, "classifier", 3), (1024, "classifier", 8),
-                                                 (16, "classifier", 8), (1024, "encoder", 8)])
+                                                 (16, "classifier", 8), (1024, "encoder", 8),
+                                                 (8192, "encoder", 8)])
 def test_repeat_hits_bitexact(features, kind, state):
     """Specificity 1: an event draws as many random literals as there are
     literals, so literals are hit two, three and four or more times by one
@@ -150,7 +151,9 @@ def test_repeat_hits_bitexact(features, kind, state):
     saturation at zero, the rounds left for a fourth hit, and the
     round-per-hit path of the four-workgroup tile (1024 features).  32 and
     2048 literals (16 and 1024 features outside the four-workgroup tile) take
-    the power-of-two draw shift, with and without tail masks."""
+    the power-of-two draw shift, with and without tail masks.  8192 features
+    is the full-wave 8-word encoder tile with its three hit levels, at 16,384
+    draws per event (256 per lane)."""
     rng = np.random.default_rng(features + state)
     n = 160
     o = dict(classes=4, features=features, clauses=16, clause_tolerance=6, clause_maximum=8, specificity=1.0,
