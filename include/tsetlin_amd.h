@@ -180,6 +180,20 @@ int tm_predict_classifier_ind(tm_machine *m, const uint8_t *problems, size_t pro
 int tm_predict_encoder_ind(tm_machine *m, const uint8_t *problems, size_t problems_bytes,
                            const int64_t *dim_offsets, uint32_t n, uint8_t *out);
 
+/* The four _ind calls above with problems (and solutions / codes, and the
+ * predictions) in device memory: the rows are repacked on the device from
+ * problems_dev, with the same dim_offsets (host) and size checks. */
+int tm_train_classifier_ind_dev(tm_machine *m, uint32_t n, const uint8_t *problems_dev, size_t problems_bytes,
+                                const int64_t *solutions_dev, const int64_t *dim_offsets, uint32_t iterations,
+                                tm_epoch_fn each, void *user);
+int tm_train_encoder_ind_dev(tm_machine *m, uint32_t n, const uint8_t *sentences_dev, size_t sentences_bytes,
+                             const uint8_t *codes_dev, const int64_t *dim_offsets, uint32_t iterations,
+                             tm_epoch_fn each, void *user);
+int tm_predict_classifier_ind_dev(tm_machine *m, const uint8_t *problems_dev, size_t problems_bytes,
+                                  const int64_t *dim_offsets, uint32_t n, int64_t *out_dev);
+int tm_predict_encoder_ind_dev(tm_machine *m, const uint8_t *problems_dev, size_t problems_bytes,
+                               const int64_t *dim_offsets, uint32_t n, uint8_t *out_dev);
+
 /* machine:restrict(selected_dims) -> tk_tsetlin_restrict, capi.c:752-900:
  * keep the classes listed in `selected` (in that order), with their actions
  * and, while the machine has them, counts.  Errors as the reference: not
@@ -290,6 +304,24 @@ int tm_last_train_stats(const tm_machine *m, uint64_t out[TM_NSTATS]);
  * *n_invalid (may be NULL). */
 int tm_bits_to_cvec_dev(const int64_t *set_bits_dev, uint64_t n_set, uint32_t n, uint32_t features, uint8_t *rows_dev,
                         uint64_t *n_invalid);
+
+/* ivec:bits_to_cvec_ind(ind_offsets, feat_offsets, n, true) of santoku-matrix
+ * (doc/sth.md:39-55; the reference's callers, classify/newsgroups.lua:97,
+ * encoder/mnist.lua:397): the packer of individualized machines' problems.
+ * ind_dev holds, for class h, the entries ind[ind_offsets[h] ..
+ * ind_offsets[h + 1]) = sample * k_h + feature (k_h = feat_offsets[h + 1] -
+ * feat_offsets[h], the feature an index into class h's own feature list), in
+ * any order, duplicates allowed; ind_offsets and feat_offsets are host arrays
+ * of classes + 1 entries starting at 0.  Class h's block, n flip-interleaved
+ * rows of ceil(2 k_h / 8) bytes as tm_bits_to_cvec_dev writes them, goes to
+ * bitmap_dev + dim_offsets[h], the blocks back to back in class order;
+ * dim_offsets (host, classes + 1 entries) receives the offsets and the total,
+ * which must not exceed bitmap_bytes (TM_E_SIZE).  Entries outside [0, n k_h)
+ * are skipped and counted in *n_invalid (may be NULL).  The bitmap and
+ * dim_offsets are what the _ind train and predict calls take. */
+int tm_bits_to_cvec_ind_dev(const int64_t *ind_dev, const int64_t *ind_offsets, const int64_t *feat_offsets,
+                            uint32_t classes, uint32_t n, uint8_t *bitmap_dev, uint64_t bitmap_bytes,
+                            int64_t *dim_offsets, uint64_t *n_invalid);
 
 /* The same rows from bit-packed ones: raw row s is ceil(features / 8) bytes,
  * bit f (LSB first) = feature f.  No reference counterpart (a dense form of

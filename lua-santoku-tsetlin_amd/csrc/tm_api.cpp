@@ -379,20 +379,26 @@ int ind_extent(const tm_machine *m, uint32_t n, const int64_t *dim_offsets, size
   return TM_OK;
 }
 
-// Stage an individualized problems buffer (host) as per-class u32 rows:
-// class h's n rows start at word woff[h], strides[h] words apart.
+// Stage an individualized problems buffer (host, or device with dev) as
+// per-class u32 rows: class h's n rows start at word woff[h], strides[h]
+// words apart.
 int stage_rows_ind(tm_machine *m, const uint8_t *rows, uint32_t n, size_t extent, const int64_t *dim_offsets,
-                   const std::vector<uint32_t> &strides, dbuf &bytes_buf, dbuf &words_buf, std::vector<uint64_t> &woff)
+                   const std::vector<uint32_t> &strides, dbuf &bytes_buf, dbuf &words_buf, std::vector<uint64_t> &woff,
+                   bool dev)
 {
   int rc;
-  if ((rc = bytes_buf.ensure(extent))) return rc;
-  HIP_TRY(hipMemcpyAsync(bytes_buf.p, rows, extent, hipMemcpyHostToDevice, m->stream));
+  const uint8_t *src = rows;
+  if (!dev) {
+    if ((rc = bytes_buf.ensure(extent))) return rc;
+    HIP_TRY(hipMemcpyAsync(bytes_buf.p, rows, extent, hipMemcpyHostToDevice, m->stream));
+    src = bytes_buf.as<uint8_t>();
+  }
   woff.assign(m->ind.size(), 0);
   uint64_t total = 0;
   for (size_t h = 0; h < m->ind.size(); h++) { woff[h] = total; total += (uint64_t) n * strides[h]; }
   if ((rc = words_buf.ensure(total * 4))) return rc;
   for (size_t h = 0; h < m->ind.size(); h++)
-    HIP_TRY(tm_launch_repack(bytes_buf.as<uint8_t>() + dim_offsets[h], n, m->ind[h].input_chunks, strides[h],
+    HIP_TRY(tm_launch_repack(src + dim_offsets[h], n, m->ind[h].input_chunks, strides[h],
                              words_buf.as<uint32_t>() + woff[h], m->stream));
   return TM_OK;
 }
@@ -472,7 +478,8 @@ int prepare_predict(tm_machine *m)
 // Class sums of an individualized machine (capi.c:1010-1029, :1057-1073): one
 // predict launch per class over that class's own rows, each adding into its
 // column of the [n][classes] sums.
-int predict_sums_ind(tm_machine *m, const uint8_t *rows, size_t bytes, const int64_t *dim_offsets, uint32_t n)
+int predict_sums_ind(tm_machine *m, const uint8_t *rows, size_t bytes, const int64_t *dim_offsets, uint32_t n,
+                     bool dev)
 {
   int rc;
   size_t extent;
@@ -482,7 +489,7 @@ int predict_sums_ind(tm_machine *m, const uint8_t *rows, size_t bytes, const int
   std::vector<uint32_t> strides(m->classes);
   for (uint32_t h = 0; h < m->classes; h++) strides[h] = m->ind[h].wd;
   std::vector<uint64_t> woff;
-  if ((rc = stage_rows_ind(m, rows, n, extent, dim_offsets, strides, m->p_rows, m->p_words, woff))) return rc;
+  if ((rc = stage_rows_ind(m, rows, n, extent, dim_offsets, strides, m->p_rows, m->p_words, woff, dev))) return rc;
   if ((rc = m->p_sums.ensure((size_t) n * m->classes * 8))) return rc;
   HIP_TRY(tm_launch_zero(m->p_sums.p, (size_t) n * m->classes * 8, m->stream));
   HIP_TRY(hipEventRecord(m->ev0, m->stream));
@@ -568,7 +575,8 @@ int predict_impl(tm_machine *m, const uint8_t *rows, uint32_t n, void *out, bool
     return fail(TM_E_ARG, "predict: dim_offsets given to a machine that is not individualized");
   if (n == 0) return TM_OK;
   if (!rows || !out) return fail(TM_E_ARG, "predict: problems and out must not be NULL");
-  if ((rc = ind_call ? predict_sums_ind(m, rows, rows_bytes, dim_offsets, n) : predict_sums(m, rows, n, dev))) return rc;
+  if ((rc = ind_call ? predict_sums_ind(m, rows, rows_bytes, dim_offsets, n, dev) : predict_sums(m, rows, n, dev)))
+    return rc;
   const size_t out_bytes = want_type == TM_CLASSIFIER ? (size_t) n * 8 : (size_t) n * m->class_chunks;
   void *dout = out;
   if (!dev) {
@@ -706,7 +714,8 @@ int train_impl(tm_machine *m, uint32_t n, const uint8_t *rows, const int64_t *la
       train_tile(m, &lanes, &wpl);
       strides.resize(m->classes);
       for (uint32_t h = 0; h < m->classes; h++) strides[h] = (m->ind[h].wd + wpl - 1) / wpl * wpl;
-      if ((rc = stage_rows_ind(m, rows, n, extent, dim_offsets, strides, m->t_rows, m->t_words, woff))) return rc;
+      if ((rc = stage_rows_ind(m, rows, n, extent, dim_offsets, strides, m->t_rows, m->t_words, woff, dev)))
+        return rc;
       words = m->t_words.as<uint32_t>();
     } else if ((rc = stage_rows(m, rows, n, dev, stride, m->t_rows, m->t_words, &words))) {
       return rc;
@@ -1081,6 +1090,34 @@ int tm_predict_encoder_ind(tm_machine *m, const uint8_t *problems, size_t proble
                            const int64_t *dim_offsets, uint32_t n, uint8_t *out)
 {
   return predict_impl(m, problems, n, out, false, TM_ENCODER, true, dim_offsets, problems_bytes);
+}
+
+int tm_train_classifier_ind_dev(tm_machine *m, uint32_t n, const uint8_t *problems_dev, size_t problems_bytes,
+                                const int64_t *solutions_dev, const int64_t *dim_offsets, uint32_t iterations,
+                                tm_epoch_fn each, void *user)
+{
+  return train_impl(m, n, problems_dev, solutions_dev, nullptr, iterations, each, user, true, TM_CLASSIFIER, true,
+                    dim_offsets, problems_bytes);
+}
+
+int tm_train_encoder_ind_dev(tm_machine *m, uint32_t n, const uint8_t *sentences_dev, size_t sentences_bytes,
+                             const uint8_t *codes_dev, const int64_t *dim_offsets, uint32_t iterations,
+                             tm_epoch_fn each, void *user)
+{
+  return train_impl(m, n, sentences_dev, nullptr, codes_dev, iterations, each, user, true, TM_ENCODER, true,
+                    dim_offsets, sentences_bytes);
+}
+
+int tm_predict_classifier_ind_dev(tm_machine *m, const uint8_t *problems_dev, size_t problems_bytes,
+                                  const int64_t *dim_offsets, uint32_t n, int64_t *out_dev)
+{
+  return predict_impl(m, problems_dev, n, out_dev, true, TM_CLASSIFIER, true, dim_offsets, problems_bytes);
+}
+
+int tm_predict_encoder_ind_dev(tm_machine *m, const uint8_t *problems_dev, size_t problems_bytes,
+                               const int64_t *dim_offsets, uint32_t n, uint8_t *out_dev)
+{
+  return predict_impl(m, problems_dev, n, out_dev, true, TM_ENCODER, true, dim_offsets, problems_bytes);
 }
 
 // capi.c:752-900: keep the classes listed in `selected`, in that order, with

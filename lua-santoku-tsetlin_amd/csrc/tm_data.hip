@@ -251,6 +251,62 @@ int tm_bits_to_cvec_dev(const int64_t *set_bits_dev, uint64_t n_set, uint32_t n,
   return TM_OK;
 }
 
+int tm_bits_to_cvec_ind_dev(const int64_t *ind_dev, const int64_t *ind_offsets, const int64_t *feat_offsets,
+                            uint32_t classes, uint32_t n, uint8_t *bitmap_dev, uint64_t bitmap_bytes,
+                            int64_t *dim_offsets, uint64_t *n_invalid)
+{
+  if (n_invalid) *n_invalid = 0;
+  if (classes == 0) return fail(TM_E_ARG, "bits_to_cvec_ind: classes must be greater than 0");
+  if (!ind_offsets || !feat_offsets || !dim_offsets) return fail(TM_E_ARG, "bits_to_cvec_ind: NULL offsets");
+  if (ind_offsets[0] != 0 || feat_offsets[0] != 0)
+    return fail(TM_E_ARG, "bits_to_cvec_ind: offsets must start at 0");
+  // class h's block: n rows of ceil(2 k_h / 8) bytes at dim_offsets[h]; in
+  // the word-aligned staging buffer it starts at word woff[h]
+  std::vector<uint64_t> rb(classes), woff(classes);
+  uint64_t total = 0, words = 0;
+  for (uint32_t h = 0; h < classes; h++) {
+    if (feat_offsets[h + 1] <= feat_offsets[h] || feat_offsets[h + 1] - feat_offsets[h] > 0x7FFFFFFF)
+      return fail(TM_E_ARG, "bits_to_cvec_ind: class %u has %lld features", h,
+                  (long long) (feat_offsets[h + 1] - feat_offsets[h]));
+    if (ind_offsets[h + 1] < ind_offsets[h])
+      return fail(TM_E_ARG, "bits_to_cvec_ind: ind_offsets decrease at class %u", h);
+    rb[h] = (2ull * (uint64_t) (feat_offsets[h + 1] - feat_offsets[h]) + 7) / 8;
+    dim_offsets[h] = (int64_t) total;
+    woff[h] = words;
+    total += (uint64_t) n * rb[h];
+    words += ((uint64_t) n * rb[h] + 3) / 4;
+  }
+  dim_offsets[classes] = (int64_t) total;
+  if (total > bitmap_bytes)
+    return fail(TM_E_SIZE, "bits_to_cvec_ind: the bitmap holds %llu bytes, the classes need %llu",
+                (unsigned long long) bitmap_bytes, (unsigned long long) total);
+  if (n == 0) return TM_OK;
+  if (!bitmap_dev || (ind_offsets[classes] && !ind_dev)) return fail(TM_E_ARG, "bits_to_cvec_ind: NULL buffer");
+  scratch stage, inv;
+  HIP_TRY(stage.get(words * 4));
+  HIP_TRY(inv.get(8));
+  HIP_TRY(hipMemsetAsync(inv.p, 0, 8, 0));
+  uint32_t *out = (uint32_t *) stage.p;
+  for (uint32_t h = 0; h < classes; h++) {
+    const uint64_t k = (uint64_t) (feat_offsets[h + 1] - feat_offsets[h]), bytes = (uint64_t) n * rb[h];
+    const uint64_t bw = (bytes + 3) / 4, cnt = (uint64_t) (ind_offsets[h + 1] - ind_offsets[h]);
+    hipLaunchKernelGGL(absent_rows_kernel, dim3(grid_for(bw, 256)), dim3(256), 0, 0, out + woff[h], bw, rb[h], k,
+                       bytes);
+    HIP_TRY(hipGetLastError());
+    if (cnt) {
+      hipLaunchKernelGGL(scatter_bits_kernel, dim3(grid_for(cnt, 256)), dim3(256), 0, 0, ind_dev + ind_offsets[h],
+                         cnt, (uint64_t) n, k, rb[h], out + woff[h], (unsigned long long *) inv.p);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(bitmap_dev + dim_offsets[h], out + woff[h], bytes, hipMemcpyDeviceToDevice, 0));
+  }
+  uint64_t bad = 0;
+  HIP_TRY(hipMemcpyAsync(&bad, inv.p, 8, hipMemcpyDeviceToHost, 0));
+  HIP_TRY(hipStreamSynchronize(0));
+  if (n_invalid) *n_invalid = bad;
+  return TM_OK;
+}
+
 int tm_flip_interleave_dev(const uint8_t *raw_dev, uint32_t n, uint32_t features, uint8_t *rows_dev)
 {
   if (features == 0) return fail(TM_E_ARG, "flip_interleave: features must be greater than 0");

@@ -142,10 +142,24 @@ class Machine:
         if self.individualized():
             if o.get("dim_offsets") is None:
                 raise TsetlinError("train: dim_offsets is required for an individualized machine")
+            da, dp = _host_buffer(o["dim_offsets"], np.int64)
+            keep += [da]
+            if _is_device_tensor(rows):
+                # problems and targets in device memory (the _ind_dev calls);
+                # the library checks dim_offsets and n against the byte size
+                rows = rows.contiguous()
+                rp = _device_buffer(rows, np.uint8, 0, "train: problems")
+                nbytes = rows.numel()
+                if enc:
+                    tp = _device_buffer(tgt, np.uint8, n * ((self.info()["classes"] + 7) // 8), "train: codes")
+                else:
+                    tp = _device_buffer(tgt, np.int64, n * 8, "train: solutions")
+                fn = self._lib.tm_train_encoder_ind_dev if enc else self._lib.tm_train_classifier_ind_dev
+                _check(fn(self._handle(), n, rp, nbytes, tp, dp, iters, cb, None))
+                return
             ra, rp = _host_buffer(rows)
             ta, tp = _host_buffer(tgt, np.uint8 if enc else np.int64)
-            da, dp = _host_buffer(o["dim_offsets"], np.int64)
-            keep += [ra, ta, da]
+            keep += [ra, ta]
             fn = self._lib.tm_train_encoder_ind if enc else self._lib.tm_train_classifier_ind
             _check(fn(self._handle(), n, rp, ra.nbytes, tp, dp, iters, cb, None))
             return
@@ -195,8 +209,21 @@ class Machine:
         return out
 
     def _predict_ind(self, problems, dim_offsets, n):
-        pa, pp = _host_buffer(problems)
         da, dp = _host_buffer(dim_offsets, np.int64)
+        if _is_device_tensor(problems):
+            import torch
+            problems = problems.contiguous()
+            pp = _device_buffer(problems, np.uint8, 0, "predict: problems")
+            if self.type() == "encoder":
+                out = torch.empty((n, (self.info()["classes"] + 7) // 8), dtype=torch.uint8, device=problems.device)
+                _check(self._lib.tm_predict_encoder_ind_dev(self._handle(), pp, problems.numel(), dp, n,
+                                                            out.data_ptr()))
+            else:
+                out = torch.empty(n, dtype=torch.int64, device=problems.device)
+                _check(self._lib.tm_predict_classifier_ind_dev(self._handle(), pp, problems.numel(), dp, n,
+                                                               out.data_ptr()))
+            return out
+        pa, pp = _host_buffer(problems)
         if self.type() == "encoder":
             out = np.zeros((n, (self.info()["classes"] + 7) // 8), np.uint8)
             _check(self._lib.tm_predict_encoder_ind(self._handle(), pp, pa.nbytes, dp, n, out.ctypes.data))

@@ -28,8 +28,9 @@ EXPORTS = [
     "tm_type", "tm_destroy", "tm_get_info", "tm_last_error", "tm_last_timing", "tm_last_train_stats",
     "tm_last_queue_timing",
     "tm_create_classifier_ind", "tm_create_encoder_ind", "tm_train_classifier_ind", "tm_train_encoder_ind",
-    "tm_predict_classifier_ind", "tm_predict_encoder_ind", "tm_restrict", "tm_get_feature_sizes",
-    "tm_bits_to_cvec_dev", "tm_flip_interleave_dev", "tm_class_accuracy_dev", "tm_encoding_accuracy_dev",
+    "tm_predict_classifier_ind", "tm_predict_encoder_ind", "tm_train_classifier_ind_dev", "tm_train_encoder_ind_dev",
+    "tm_predict_classifier_ind_dev", "tm_predict_encoder_ind_dev", "tm_restrict", "tm_get_feature_sizes",
+    "tm_bits_to_cvec_dev", "tm_bits_to_cvec_ind_dev", "tm_flip_interleave_dev", "tm_class_accuracy_dev", "tm_encoding_accuracy_dev",
     "tm_batch_create", "tm_batch_machine", "tm_batch_train_classifier", "tm_batch_train_classifier_dev",
     "tm_batch_accuracy_dev", "tm_batch_class_accuracy_dev", "tm_batch_tuning", "tm_batch_timing", "tm_batch_destroy",
 ]
@@ -118,9 +119,15 @@ def load_library(path=LIB_PATH):
         "tm_train_encoder_ind": (ctypes.c_int, [vp, u32, u8p, ctypes.c_size_t, u8p, i64p, u32, EPOCH_FN, vp]),
         "tm_predict_classifier_ind": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, i64p, u32, i64p]),
         "tm_predict_encoder_ind": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, i64p, u32, u8p]),
+        "tm_train_classifier_ind_dev": (ctypes.c_int, [vp, u32, vp, ctypes.c_size_t, vp, i64p, u32, EPOCH_FN, vp]),
+        "tm_train_encoder_ind_dev": (ctypes.c_int, [vp, u32, vp, ctypes.c_size_t, vp, i64p, u32, EPOCH_FN, vp]),
+        "tm_predict_classifier_ind_dev": (ctypes.c_int, [vp, vp, ctypes.c_size_t, i64p, u32, vp]),
+        "tm_predict_encoder_ind_dev": (ctypes.c_int, [vp, vp, ctypes.c_size_t, i64p, u32, vp]),
         "tm_restrict": (ctypes.c_int, [vp, i64p, u32]),
         "tm_get_feature_sizes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64), u32]),
         "tm_bits_to_cvec_dev": (ctypes.c_int, [vp, ctypes.c_uint64, u32, u32, vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "tm_bits_to_cvec_ind_dev": (ctypes.c_int, [vp, i64p, i64p, u32, u32, vp, ctypes.c_uint64, i64p,
+                                                   ctypes.POINTER(ctypes.c_uint64)]),
         "tm_flip_interleave_dev": (ctypes.c_int, [vp, u32, u32, vp]),
         "tm_class_accuracy_dev": (ctypes.c_int, [vp, vp, u32, u32, ctypes.POINTER(ctypes.c_double),
                                                  ctypes.POINTER(ctypes.c_double)]),

@@ -16,6 +16,73 @@ def bits_to_cvec(x):
     return np.packbits(lits, axis=1, bitorder="little")
 
 
+def bits_individualize(set_bits, feat_offsets, feat_ids, union_size):
+    """toks:bits_individualize(offsets, ids, union_size) of santoku-matrix, as
+    the reference's callers use it (doc/sth.md:38; classify/newsgroups.lua:96,
+    encoder/mnist.lua:396): set_bits holds sample * union_size + f for each
+    present feature f of the union vocabulary; class h's features are
+    feat_ids[feat_offsets[h] : feat_offsets[h + 1]] (indices into the union,
+    distinct within a class).  Returns (ind, ind_offsets): class h's entries
+    ind[ind_offsets[h] : ind_offsets[h + 1]] = sample * k_h + i for every
+    present feature feat_ids[feat_offsets[h] + i], sorted, and the classes + 1
+    offsets.  Entries outside [0, n * union_size) are the caller's error:
+    negative ones raise, others map to samples past n and are skipped by the
+    packers' range check."""
+    set_bits = np.asarray(set_bits, np.int64).ravel()
+    feat_offsets = np.asarray(feat_offsets, np.int64).ravel()
+    feat_ids = np.asarray(feat_ids, np.int64).ravel()
+    if union_size <= 0:
+        raise ValueError("bits_individualize: union_size must be positive")
+    if (set_bits < 0).any():
+        raise ValueError("bits_individualize: negative set bit")
+    s, f = set_bits // union_size, set_bits % union_size
+    out, offs = [], [0]
+    for h in range(feat_offsets.size - 1):
+        ids = feat_ids[feat_offsets[h]:feat_offsets[h + 1]]
+        if np.unique(ids).size != ids.size:
+            raise ValueError("bits_individualize: class %d lists a feature twice" % h)
+        if ids.size and (ids.min() < 0 or ids.max() >= union_size):
+            raise ValueError("bits_individualize: class %d lists a feature outside the union" % h)
+        lut = np.full(union_size, -1, np.int64)
+        lut[ids] = np.arange(ids.size)
+        loc = lut[f]
+        keep = loc >= 0
+        e = np.unique(s[keep] * ids.size + loc[keep])
+        out.append(e)
+        offs.append(offs[-1] + e.size)
+    return (np.concatenate(out) if out else np.zeros(0, np.int64)), np.asarray(offs, np.int64)
+
+
+def bits_to_cvec_ind(ind, ind_offsets, feat_offsets, n, flip=True):
+    """ivec:bits_to_cvec_ind(ind_offsets, feat_offsets, n, flip) of
+    santoku-matrix (doc/sth.md:39-55): class h's entries (sample * k_h +
+    feature, k_h = feat_offsets[h + 1] - feat_offsets[h]) packed into n rows
+    of ceil(2 k_h / 8) bytes, flip-interleaved as bits_to_cvec (ceil(k_h / 8)
+    bytes of x bits without flip), the classes' blocks back to back.  Returns
+    (bitmap, dim_offsets): a uint8 array and the classes + 1 byte offsets of
+    the blocks (the last one the total), which the individualized train and
+    predict calls take.  Entries outside [0, n * k_h) are skipped."""
+    ind = np.asarray(ind, np.int64).ravel()
+    ind_offsets = np.asarray(ind_offsets, np.int64).ravel()
+    feat_offsets = np.asarray(feat_offsets, np.int64).ravel()
+    classes = feat_offsets.size - 1
+    if classes < 1 or ind_offsets.size != classes + 1:
+        raise ValueError("bits_to_cvec_ind: offsets must have classes + 1 entries")
+    blocks, dim = [], [0]
+    for h in range(classes):
+        k = int(feat_offsets[h + 1] - feat_offsets[h])
+        if k <= 0:
+            raise ValueError("bits_to_cvec_ind: class %d has no features" % h)
+        e = ind[ind_offsets[h]:ind_offsets[h + 1]]
+        e = e[(e >= 0) & (e < n * k)]
+        x = np.zeros((n, k), bool)
+        x[e // k, e % k] = True
+        rows = bits_to_cvec(x) if flip else np.packbits(x, axis=1, bitorder="little")
+        blocks.append(rows.reshape(-1))
+        dim.append(dim[-1] + rows.size)
+    return np.concatenate(blocks), np.asarray(dim, np.int64)
+
+
 def codes_from_bits(h):
     """h: (n, H) bool -> (n, ceil(H/8)) uint8 codes, LSB first (capi.c:1254)."""
     return np.packbits(np.asarray(h).astype(bool, copy=False), axis=1, bitorder="little")
@@ -149,6 +216,36 @@ def bits_to_cvec_dev(set_bits, n, features, out=None):
     bad = ctypes.c_uint64(0)
     _dev_check(_dev_lib().tm_bits_to_cvec_dev(sp, set_bits.numel(), n, features, op, ctypes.byref(bad)))
     return out, bad.value
+
+
+def bits_to_cvec_ind_dev(ind, ind_offsets, feat_offsets, n, out=None):
+    """bits_to_cvec_ind(ind_offsets, feat_offsets, n, true) on the device: ind
+    is an int64 GPU tensor of class-major entries (sample * k_h + feature),
+    ind_offsets and feat_offsets host arrays of classes + 1 entries.  Returns
+    (bitmap, dim_offsets, invalid): a uint8 GPU tensor, the classes + 1 byte
+    offsets (numpy int64) and the count of skipped out-of-range entries."""
+    import ctypes
+    import torch
+    from . import _device_buffer
+    ind = ind.contiguous()
+    io = np.ascontiguousarray(ind_offsets, np.int64)
+    fo = np.ascontiguousarray(feat_offsets, np.int64)
+    classes = fo.size - 1
+    if classes < 1 or io.size != classes + 1:
+        raise ValueError("bits_to_cvec_ind: offsets must have classes + 1 entries")
+    if io[-1] > ind.numel():
+        raise ValueError("bits_to_cvec_ind: ind_offsets run past the %d entries of ind" % ind.numel())
+    k = np.diff(fo)
+    total = int((n * ((2 * k + 7) // 8)).sum()) if (k > 0).all() else 0
+    ip = _device_buffer(ind, np.int64, int(io[-1]) * 8, "bits_to_cvec_ind: ind")
+    if out is None:
+        out = torch.empty(max(total, 1), dtype=torch.uint8, device=ind.device)
+    op = _device_buffer(out, np.uint8, total, "bits_to_cvec_ind: out")
+    dim = np.zeros(classes + 1, np.int64)
+    bad = ctypes.c_uint64(0)
+    _dev_check(_dev_lib().tm_bits_to_cvec_ind_dev(ip, io.ctypes.data, fo.ctypes.data, classes, n, op,
+                                                  out.numel(), dim.ctypes.data, ctypes.byref(bad)))
+    return out, dim, bad.value
 
 
 def flip_interleave_dev(raw, n, features, out=None):

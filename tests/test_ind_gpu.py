@@ -281,3 +281,80 @@ def test_ind_zero_samples_and_iterations():
     assert eng.predict(prob, offs, 0).shape == (0,)
     enc, _ = pair("encoder", widths)
     assert enc.predict(prob, offs, 0).shape == (0, 1)
+
+
+# ---------------------------------------------------------------------------
+# Device-resident individualized data: the packer and the _ind_dev calls.
+
+def test_bits_to_cvec_ind_dev_equals_host_packer():
+    """tm_bits_to_cvec_ind_dev against the host restatement: ragged widths
+    (rows of 1 to 75 bytes, so most class blocks start off a word boundary),
+    an odd sample count, duplicates, out-of-range entries (counted, skipped),
+    a class without entries, and an output tensor whose start is not word
+    aligned."""
+    import torch
+    rng = np.random.default_rng(21)
+    n, union = 37, 500
+    x = rng.random((n, union)) < 0.15
+    widths = [1, 300, 5, 64, 9, 17]
+    ids = [rng.choice(union, k, replace=False) for k in widths]
+    fo = np.concatenate([[0], np.cumsum(widths)]).astype(np.int64)
+    s, f = np.nonzero(x)
+    sb = np.concatenate([s * union + f, (s * union + f)[:40]])
+    ind, io = D.bits_individualize(sb, fo, np.concatenate(ids), union)
+    io = io.copy()
+    # class 3 gets no entries; class 1 gets two out-of-range ones
+    ind = np.concatenate([ind[:io[1]], ind[io[1]:io[2]], [-3, n * widths[1]], ind[io[2]:io[3]], ind[io[4]:]])
+    sizes = np.diff(io)
+    sizes[1] += 2
+    sizes[3] = 0
+    io = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    want, wdim = D.bits_to_cvec_ind(ind, io, fo, n)
+    total = int(wdim[-1])
+    big = torch.full((total + 5,), 0xA5, dtype=torch.uint8, device="cuda")
+    out = big[1:1 + total]
+    got, dim, bad = D.bits_to_cvec_ind_dev(torch.from_numpy(ind).cuda(), io, fo, n, out=out)
+    assert bad == 2
+    assert np.array_equal(dim, wdim)
+    assert np.array_equal(got.cpu().numpy(), want)
+    b = big.cpu().numpy()
+    assert b[0] == 0xA5 and (b[1 + total:] == 0xA5).all()     # nothing written outside the output
+
+
+@pytest.mark.parametrize("kind", ["classifier", "encoder"])
+def test_ind_dev_calls_equal_host_calls(kind):
+    """Pack on the device, then train and predict with the problems in device
+    memory (the _ind_dev calls): the state and the predictions equal the host
+    calls' on the host-packed bytes, bit for bit."""
+    import torch
+    rng = np.random.default_rng(33 if kind == "classifier" else 34)
+    n, union = 140, 300
+    x = rng.random((n, union)) < 0.25
+    widths = [40, 5, 120, 64] if kind == "classifier" else [40, 5, 120, 64, 9, 200, 31, 2, 77]
+    K = len(widths)
+    ids = [rng.choice(union, k, replace=False) for k in widths]
+    fo = np.concatenate([[0], np.cumsum(widths)]).astype(np.int64)
+    s, f = np.nonzero(x)
+    ind, io = D.bits_individualize(s * union + f, fo, np.concatenate(ids), union)
+    hb, hdim = D.bits_to_cvec_ind(ind, io, fo, n)
+    db, ddim, bad = D.bits_to_cvec_ind_dev(torch.from_numpy(ind).cuda(), io, fo, n)
+    assert bad == 0 and np.array_equal(ddim, hdim) and np.array_equal(db.cpu().numpy(), hb)
+    o = dict(classes=K, individualized=True, feat_offsets=fo, clauses=16, clause_tolerance=6, clause_maximum=8,
+             specificity=3.0, target=8, seed=5, reusable=True)
+    host, dev = tm.create(kind, dict(o)), tm.create(kind, dict(o))
+    if kind == "classifier":
+        y = rng.integers(0, K, n).astype(np.int64)
+        host.train(samples=n, problems=hb, solutions=y, dim_offsets=hdim[:K], iterations=3)
+        dev.train(samples=n, problems=db, solutions=torch.from_numpy(y).cuda(), dim_offsets=ddim[:K], iterations=3)
+    else:
+        codes = rng.integers(0, 256, (n, (K + 7) // 8), dtype=np.uint8)
+        host.train(samples=n, sentences=hb, codes=codes, dim_offsets=hdim[:K], iterations=3)
+        dev.train(samples=n, sentences=db, codes=torch.from_numpy(codes).cuda(), dim_offsets=ddim[:K], iterations=3)
+    assert np.array_equal(host.checkpoint(), dev.checkpoint())
+    assert np.array_equal(host.counts(), dev.counts())
+    want = host.predict(hb, hdim[:K], n)
+    got = dev.predict(db, ddim[:K], n)
+    assert np.array_equal(got.cpu().numpy(), want)
+    # the byte count the library checks is the tensor's: a short one is refused
+    with pytest.raises(tm.TsetlinError):
+        dev.predict(db[:int(ddim[-1]) - 1], ddim[:K], n)

@@ -113,3 +113,65 @@ def test_class_params():
     O.lib().orc_ind_class(ctypes.byref(p), 13, ctypes.byref(ph))
     assert ph.features == 13 and ph.input_bits == 26 and ph.input_chunks == 4 and ph.tail_mask == 0x03
     assert ph.target == p.target and ph.negative == pytest.approx(0.2) and ph.clauses == 32
+
+
+# ---------------------------------------------------------------------------
+# The individualized packers (santoku-matrix's bits_individualize and
+# bits_to_cvec_ind, restated from their call sites and doc/sth.md:30-91).
+
+def test_ind_packers_hand_example():
+    """Union of 3 features; class 0 lists union features [2, 0], class 1 [1].
+    Sample 0 has features {0, 2}, sample 1 has {1}.  Class 0 (k = 2, one byte
+    per row): sample 0 has x = (1, 1), bits 0-1 set -> 0x03; sample 1 has
+    x = (0, 0), the not-x bits 2-3 set -> 0x0C.  Class 1 (k = 1): sample 0
+    absent -> not-x bit 1 -> 0x02; sample 1 present -> 0x01."""
+    from santoku.tsetlin import data as D
+    union = 3
+    set_bits = np.array([0 * union + 0, 0 * union + 2, 1 * union + 1])
+    ind, ind_off = D.bits_individualize(set_bits, [0, 2, 3], [2, 0, 1], union)
+    assert ind.tolist() == [0, 1, 1] and ind_off.tolist() == [0, 2, 3]
+    bitmap, dim = D.bits_to_cvec_ind(ind, ind_off, [0, 2, 3], 2)
+    assert bitmap.tolist() == [0x03, 0x0C, 0x02, 0x01] and dim.tolist() == [0, 2, 4]
+
+
+def test_ind_packers_equal_the_plain_packer_on_selected_columns():
+    """Class h's block is the plain packer over the union columns it selects,
+    in its order; blocks follow each other in class order.  Ragged widths
+    (1 to 300 features, rows that are not whole words), duplicates in the
+    input and samples without any feature."""
+    from santoku.tsetlin import data as D
+    rng = np.random.default_rng(4)
+    n, union = 23, 400
+    x = rng.random((n, union)) < 0.2
+    x[5] = False
+    widths = [1, 7, 300, 64, 13]
+    ids = [rng.choice(union, k, replace=False) for k in widths]
+    feat_offsets = np.concatenate([[0], np.cumsum(widths)])
+    s, f = np.nonzero(x)
+    set_bits = np.concatenate([s * union + f, (s * union + f)[:50]])      # with duplicates
+    rng.shuffle(set_bits)
+    ind, ind_off = D.bits_individualize(set_bits, feat_offsets, np.concatenate(ids), union)
+    bitmap, dim = D.bits_to_cvec_ind(ind, ind_off, feat_offsets, n)
+    assert dim[0] == 0 and dim[-1] == bitmap.size
+    for h, k in enumerate(widths):
+        want = D.bits_to_cvec(x[:, ids[h]]).reshape(-1)
+        assert dim[h + 1] - dim[h] == n * ((2 * k + 7) // 8)
+        assert np.array_equal(bitmap[dim[h]:dim[h + 1]], want), h
+    # without flip: the x bits only
+    raw, rdim = D.bits_to_cvec_ind(ind, ind_off, feat_offsets, n, flip=False)
+    for h, k in enumerate(widths):
+        want = np.packbits(x[:, ids[h]], axis=1, bitorder="little").reshape(-1)
+        assert np.array_equal(raw[rdim[h]:rdim[h + 1]], want), h
+
+
+def test_ind_packers_argument_errors():
+    from santoku.tsetlin import data as D
+    with pytest.raises(ValueError):
+        D.bits_individualize([0, 1], [0, 2], [1, 1], 4)          # a feature listed twice
+    with pytest.raises(ValueError):
+        D.bits_individualize([0, 1], [0, 1], [9], 4)             # outside the union
+    with pytest.raises(ValueError):
+        D.bits_to_cvec_ind([0], [0, 1], [0, 0], 1)               # an empty class
+    # entries past the rows are skipped
+    bitmap, dim = D.bits_to_cvec_ind([0, 5, -1], [0, 3], [0, 2], 1)
+    assert bitmap.tolist() == [0x01 | 0x08] and dim.tolist() == [0, 1]
