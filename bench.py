@@ -294,7 +294,7 @@ SEARCH_SPACE = {
 }
 
 
-def search_measure(seed=1, search_seeds=3):
+def search_measure(seed=1, search_seeds=8):
     """Workloads 4 and 5 on workload 1's data (8192 train / 2048 dev rows, 256
     features, 4 classes, 10 epochs per trial).
 
@@ -385,10 +385,19 @@ def search_measure(seed=1, search_seeds=3):
             line.append((time.perf_counter() - t_start, best[0]))
             return sc
 
-        prop = S.Proposer(ctx, seed=sseed) if surrogate else None
+        prop_s = [0.0]
+        prop = None
+        if surrogate:
+            net = S.Proposer(ctx, seed=sseed)
+
+            def prop(samplers, n, history):
+                t0 = time.perf_counter()
+                out = net(samplers, n, history)
+                prop_s[0] += time.perf_counter() - t0
+                return out
         _, score, _ = opt.search_batched(opt.PARAM_NAMES, smp, timed, rounds=8, trials=64, propose=prop,
                                          make_key=opt.make_key)
-        return line, score
+        return line, score, prop_s[0]
 
     def time_to(line, target):
         for t, b_ in line:
@@ -398,14 +407,14 @@ def search_measure(seed=1, search_seeds=3):
 
     results = []
     for sseed in range(seed, seed + search_seeds):
-        p_line, p_best = run_search(False, sseed)
-        s_line, s_best = run_search(True, sseed)
+        p_line, p_best, _ = run_search(False, sseed)
+        s_line, s_best, s_prop = run_search(True, sseed)
         target = p_best - 0.005
         tp, ts = time_to(p_line, target), time_to(s_line, target)
         results.append({"seed": sseed, "plain_best_f1": round(p_best, 4), "surrogate_best_f1": round(s_best, 4),
                         "plain_time_to_target_s": round(tp, 2), "plain_total_s": round(p_line[-1][0], 2),
                         "surrogate_time_to_target_s": None if ts is None else round(ts, 2),
-                        "surrogate_total_s": round(s_line[-1][0], 2),
+                        "surrogate_total_s": round(s_line[-1][0], 2), "surrogate_proposer_s": round(s_prop, 2),
                         "ratio_vs_plain_time_to_target": None if ts is None else round(ts / tp, 3),
                         "ratio_vs_plain_total": None if ts is None else round(ts / p_line[-1][0], 3)})
         log("workload 5 seed %d: %s" % (sseed, results[-1]))
@@ -415,6 +424,10 @@ def search_measure(seed=1, search_seeds=3):
                     "target = plain search's best - 0.5 pt",
         "seeds": results,
         "reached_target": "%d/%d" % (len(reached), len(results)),
+        "surrogate_first": "%d/%d" % (sum(1 for r in reached if r["ratio_vs_plain_time_to_target"] < 1.0),
+                                      len(results)),
+        "surrogate_best_above_plain_best": "%d/%d" % (sum(1 for r in results
+                                                          if r["surrogate_best_f1"] > r["plain_best_f1"]), len(results)),
         "mean_ratio_vs_plain_total": round(float(np.mean([r["ratio_vs_plain_total"] for r in reached])), 3)
         if reached else None,
         "mean_ratio_vs_plain_time_to_target": round(float(np.mean([r["ratio_vs_plain_time_to_target"]
@@ -433,7 +446,7 @@ def main():
     ap.add_argument("--n-test", type=int, default=20_000)
     ap.add_argument("--cpu-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-search", action="store_true", help="skip workloads 4 and 5 (batched trials, surrogate search)")
-    ap.add_argument("--search-seeds", type=int, default=3)
+    ap.add_argument("--search-seeds", type=int, default=8)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--encoder", action="store_true", help="workload 3: the 256-bit encoder at full machine shape")
     ap.add_argument("--enc-train", type=int, default=16_384)

@@ -5,9 +5,9 @@
 //
 // Parameters are one flat float array (9217 values):
 //   W1[64][12] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4
-// The network is tiny, so one workgroup trains it: weights and gradients sit in
-// LDS (2 x 36.9 KB) and per-sample activations and deltas in a global scratch
-// area that stays in L2.
+// Training splits the samples into slices of 32, one workgroup each, with the
+// weights (also transposed), activations and deltas of its slice in LDS; the
+// slices' gradient sums meet in a second kernel that applies Adam.
 #include "tsetlin_amd.h"
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -81,137 +81,161 @@ __global__ __launch_bounds__(MLP_THREADS) void mlp_predict_kernel(const float *_
   Y[s] = forward_one(w, x, h1, h2, h3);
 }
 
-struct mlp_train_args {
-  float *params, *m, *v;         // weights and Adam moments (global)
-  const float *X, *Y;            // n samples
-  float *act;                    // scratch: A1, A2, A3, D1, D2, D3 [n][64] and dout [n]
-  float *grad_out;               // mode 1: the loss gradient is written here, nothing updated
-  float *loss_out;               // loss of the last step (mean squared error)
-  uint32_t n, steps, step0, mode;
+// Training is full-batch mean-squared error with Adam, one step at a time:
+//   mlp_grad_kernel: one workgroup per slice of MLP_SLICE samples runs the
+//     forward and backward passes of its slice in LDS and writes the slice's
+//     gradient sums and squared-error sum;
+//   mlp_adam_kernel: one thread per parameter adds the slices' sums and
+//     applies the Adam update (or, for a gradient evaluation, stores the sum).
+// A step is the two launches, so every workgroup of the gradient pass reads
+// the weights the previous step wrote.
+constexpr int MLP_SLICE = 32;
+constexpr int HP = H + 1;     // activation row stride: rows of consecutive samples fall in different LDS banks
+
+struct mlp_grad_args {
+  const float *params;
+  const float *X, *Y;
+  float *part;                 // [slices][NP] gradient sums
+  float *lpart;                // [slices] squared-error sums
+  uint32_t n;
+};
+
+// out[s][i] = relu(b[i] + sum_k W[i][k] in[s][k]) with the weights transposed
+// (wt[k][i]): the lanes of a wave take consecutive units i of one sample, so
+// the weight reads are consecutive and the input read is a broadcast
+__device__ __forceinline__ void fwd_layer(const float *wt, const float *b, int nin, const float *in, int in_stride,
+                                          float *out, int tid)
+{
+  for (int idx = tid; idx < MLP_SLICE * H; idx += MLP_THREADS) {
+    const int s = idx / H, i = idx % H;
+    float a = b[i];
+    for (int k = 0; k < nin; k++) a = fmaf(wt[k * H + i], in[s * in_stride + k], a);
+    out[s * HP + i] = fmaxf(a, 0.f);
+  }
+}
+
+// d[s][i] = (a[s][i] > 0) * sum_k W[k][i] dn[s][k], W in its own layout [k][i]
+__device__ __forceinline__ void back_layer(const float *w, const float *dn, const float *a, float *d, int tid)
+{
+  for (int idx = tid; idx < MLP_SLICE * H; idx += MLP_THREADS) {
+    const int s = idx / H, i = idx % H;
+    float acc = 0.f;
+    for (int k = 0; k < H; k++) acc = fmaf(w[k * H + i], dn[s * HP + k], acc);
+    d[s * HP + i] = a[s * HP + i] > 0.f ? acc : 0.f;
+  }
+}
+
+__global__ __launch_bounds__(MLP_THREADS) void mlp_grad_kernel(mlp_grad_args A)
+{
+  __shared__ float w[NP];
+  __shared__ float wt[IN * H + 2 * H * H];          // W1, W2, W3 transposed
+  __shared__ float xs[MLP_SLICE * IN];
+  __shared__ float act[3][MLP_SLICE * HP];          // post-ReLU activations of the three hidden layers
+  __shared__ float del[3][MLP_SLICE * HP];          // their deltas
+  __shared__ float dout[MLP_SLICE];
+  const int tid = threadIdx.x;
+  const uint32_t s0 = blockIdx.x * MLP_SLICE;
+  const uint32_t ns = min((uint32_t) MLP_SLICE, A.n - s0);
+  for (int p = tid; p < NP; p += MLP_THREADS) {
+    const float v = A.params[p];
+    w[p] = v;
+    if (p < OB1) wt[(p % IN) * H + p / IN] = v;
+    else if (p >= OW2 && p < OB2) wt[IN * H + ((p - OW2) % H) * H + (p - OW2) / H] = v;
+    else if (p >= OW3 && p < OB3) wt[IN * H + H * H + ((p - OW3) % H) * H + (p - OW3) / H] = v;
+  }
+  // samples past the slice's end enter as zeros and leave with zero deltas
+  for (int idx = tid; idx < MLP_SLICE * IN; idx += MLP_THREADS)
+    xs[idx] = (uint32_t)(idx / IN) < ns ? A.X[(size_t) s0 * IN + idx] : 0.f;
+  __syncthreads();
+  fwd_layer(wt, w + OB1, IN, xs, IN, act[0], tid);
+  __syncthreads();
+  fwd_layer(wt + IN * H, w + OB2, H, act[0], HP, act[1], tid);
+  __syncthreads();
+  fwd_layer(wt + IN * H + H * H, w + OB3, H, act[1], HP, act[2], tid);
+  __syncthreads();
+  float l = 0.f;
+  if (tid < MLP_SLICE) {     // wave 0: one sample per lane
+    float o = w[OB4];
+    for (int k = 0; k < H; k++) o = fmaf(w[OW4 + k], act[2][tid * HP + k], o);
+    const float err = (uint32_t) tid < ns ? o - A.Y[s0 + tid] : 0.f;
+    l = err * err;
+    dout[tid] = 2.f * err / (float) A.n;
+  }
+  if (tid < 64) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+    if (tid == 0) A.lpart[blockIdx.x] = l;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < MLP_SLICE * H; idx += MLP_THREADS) {
+    const int s = idx / H, i = idx % H;
+    del[2][s * HP + i] = act[2][s * HP + i] > 0.f ? dout[s] * w[OW4 + i] : 0.f;
+  }
+  __syncthreads();
+  back_layer(w + OW3, del[2], act[1], del[1], tid);
+  __syncthreads();
+  back_layer(w + OW2, del[1], act[0], del[0], tid);
+  __syncthreads();
+  // the slice's gradient sums, one parameter per thread at a time
+  float *part = A.part + (size_t) blockIdx.x * NP;
+  for (int p = tid; p < NP; p += MLP_THREADS) {
+    float acc = 0.f;
+    if (p < OB1) {
+      const int i = p / IN, k = p % IN;
+      for (int s = 0; s < MLP_SLICE; s++) acc = fmaf(del[0][s * HP + i], xs[s * IN + k], acc);
+    } else if (p < OW2) {
+      for (int s = 0; s < MLP_SLICE; s++) acc += del[0][s * HP + p - OB1];
+    } else if (p < OB2) {
+      const int i = (p - OW2) / H, k = (p - OW2) % H;
+      for (int s = 0; s < MLP_SLICE; s++) acc = fmaf(del[1][s * HP + i], act[0][s * HP + k], acc);
+    } else if (p < OW3) {
+      for (int s = 0; s < MLP_SLICE; s++) acc += del[1][s * HP + p - OB2];
+    } else if (p < OB3) {
+      const int i = (p - OW3) / H, k = (p - OW3) % H;
+      for (int s = 0; s < MLP_SLICE; s++) acc = fmaf(del[2][s * HP + i], act[1][s * HP + k], acc);
+    } else if (p < OW4) {
+      for (int s = 0; s < MLP_SLICE; s++) acc += del[2][s * HP + p - OB3];
+    } else if (p < OB4) {
+      for (int s = 0; s < MLP_SLICE; s++) acc = fmaf(dout[s], act[2][s * HP + p - OW4], acc);
+    } else {
+      for (int s = 0; s < MLP_SLICE; s++) acc += dout[s];
+    }
+    part[p] = acc;
+  }
+}
+
+struct mlp_adam_args {
+  float *params, *m, *v;       // weights and Adam moments
+  const float *part, *lpart;   // the gradient pass's slice sums
+  float *grad_out;             // gradient evaluation: the summed gradient goes here, nothing is updated
+  float *loss_out;             // mean squared error of the step's forward pass
+  uint32_t slices, n, t;       // t: Adam step number, from 1
   float lr, beta1, beta2, eps;
 };
 
-// `steps` Adam steps of full-batch mean-squared-error training in one
-// workgroup (mode 0), or one gradient evaluation (mode 1).  Every layer is a
-// small matrix product: one thread per (sample, unit) output for the forward
-// and delta passes, one thread per parameter for the gradient sums, with a
-// workgroup barrier between passes.
-__device__ __forceinline__ void dense_relu(const float *__restrict__ w, int ow, int ob, int nin, const float *in,
-                                           float *out, uint32_t n, int tid)
+__global__ __launch_bounds__(MLP_THREADS) void mlp_adam_kernel(mlp_adam_args A)
 {
-  for (uint32_t idx = tid; idx < n * H; idx += MLP_THREADS) {
-    const uint32_t s = idx / H, i = idx % H;
-    float a = w[ob + i];
-    const float *x = in + (size_t) s * nin;
-    for (int k = 0; k < nin; k++) a = fmaf(w[ow + i * nin + k], x[k], a);
-    out[idx] = fmaxf(a, 0.f);
+  const uint32_t p = blockIdx.x * MLP_THREADS + threadIdx.x;
+  if (p == 0 && A.loss_out) {
+    float l = 0.f;
+    for (uint32_t b = 0; b < A.slices; b++) l += A.lpart[b];
+    *A.loss_out = l / (float) A.n;
   }
-}
-
-// delta of layer l from the delta of layer l+1: D[s][i] = (A[s][i] > 0) * sum_k W[k][i] Dn[s][k]
-__device__ __forceinline__ void dense_back(const float *__restrict__ w, int ow, const float *dn, const float *a,
-                                           float *d, uint32_t n, int tid)
-{
-  for (uint32_t idx = tid; idx < n * H; idx += MLP_THREADS) {
-    const uint32_t s = idx / H, i = idx % H;
-    float acc = 0.f;
-    const float *dr = dn + (size_t) s * H;
-    for (int k = 0; k < H; k++) acc = fmaf(w[ow + k * H + i], dr[k], acc);
-    d[idx] = a[idx] > 0.f ? acc : 0.f;
+  if (p >= (uint32_t) NP) return;
+  float g = 0.f;
+  for (uint32_t b = 0; b < A.slices; b++) g += A.part[(size_t) b * NP + p];
+  if (A.grad_out) {
+    A.grad_out[p] = g;
+    return;
   }
-}
-
-__global__ __launch_bounds__(MLP_THREADS) void mlp_train_kernel(mlp_train_args A)
-{
-  __shared__ float w[NP];
-  __shared__ float g[NP];
-  __shared__ float red[MLP_THREADS];
-  const int tid = threadIdx.x;
-  const uint32_t n = A.n;
-  float *A1 = A.act, *A2 = A1 + (size_t) n * H, *A3 = A2 + (size_t) n * H;
-  float *D1 = A3 + (size_t) n * H, *D2 = D1 + (size_t) n * H, *D3 = D2 + (size_t) n * H, *DO = D3 + (size_t) n * H;
-  for (int i = tid; i < NP; i += MLP_THREADS) w[i] = A.params[i];
-  __syncthreads();
-  const uint32_t nsteps = A.mode == 1 ? 1 : A.steps;
-  for (uint32_t step = 0; step < nsteps; step++) {
-    dense_relu(w, OW1, OB1, IN, A.X, A1, n, tid);
-    __syncthreads();
-    dense_relu(w, OW2, OB2, H, A1, A2, n, tid);
-    __syncthreads();
-    dense_relu(w, OW3, OB3, H, A2, A3, n, tid);
-    __syncthreads();
-    float lsum = 0.f;
-    for (uint32_t s = tid; s < n; s += MLP_THREADS) {
-      float o = w[OB4];
-      for (int k = 0; k < H; k++) o = fmaf(w[OW4 + k], A3[(size_t) s * H + k], o);
-      const float err = o - A.Y[s];
-      lsum += err * err;
-      DO[s] = 2.f * err / (float) n;
-    }
-    red[tid] = lsum;
-    __syncthreads();
-    for (uint32_t idx = tid; idx < n * H; idx += MLP_THREADS) {
-      const uint32_t s = idx / H, i = idx % H;
-      D3[idx] = A3[idx] > 0.f ? DO[s] * w[OW4 + i] : 0.f;
-    }
-    __syncthreads();
-    dense_back(w, OW3, D3, A2, D2, n, tid);
-    __syncthreads();
-    dense_back(w, OW2, D2, A1, D1, n, tid);
-    __syncthreads();
-    // gradients: each thread owns a strided set of parameters and sums over samples
-    for (int p = tid; p < NP; p += MLP_THREADS) {
-      float acc = 0.f;
-      if (p < OB1) {
-        const int i = (p - OW1) / IN, k = (p - OW1) % IN;
-        for (uint32_t s = 0; s < n; s++) acc = fmaf(D1[(size_t) s * H + i], A.X[(size_t) s * IN + k], acc);
-      } else if (p < OW2) {
-        const int i = p - OB1;
-        for (uint32_t s = 0; s < n; s++) acc += D1[(size_t) s * H + i];
-      } else if (p < OB2) {
-        const int i = (p - OW2) / H, k = (p - OW2) % H;
-        for (uint32_t s = 0; s < n; s++) acc = fmaf(D2[(size_t) s * H + i], A1[(size_t) s * H + k], acc);
-      } else if (p < OW3) {
-        const int i = p - OB2;
-        for (uint32_t s = 0; s < n; s++) acc += D2[(size_t) s * H + i];
-      } else if (p < OB3) {
-        const int i = (p - OW3) / H, k = (p - OW3) % H;
-        for (uint32_t s = 0; s < n; s++) acc = fmaf(D3[(size_t) s * H + i], A2[(size_t) s * H + k], acc);
-      } else if (p < OW4) {
-        const int i = p - OB3;
-        for (uint32_t s = 0; s < n; s++) acc += D3[(size_t) s * H + i];
-      } else if (p < OB4) {
-        const int k = p - OW4;
-        for (uint32_t s = 0; s < n; s++) acc = fmaf(DO[s], A3[(size_t) s * H + k], acc);
-      } else {
-        for (uint32_t s = 0; s < n; s++) acc += DO[s];
-      }
-      g[p] = acc;
-    }
-    __syncthreads();
-    if (tid == 0 && A.loss_out) {
-      float l = 0.f;
-      for (int t = 0; t < MLP_THREADS; t++) l += red[t];
-      *A.loss_out = l / (float) n;
-    }
-    if (A.mode == 1) {
-      for (int p = tid; p < NP; p += MLP_THREADS) A.grad_out[p] = g[p];
-      return;
-    }
-    // Adam (Kingma & Ba, 2015) with bias correction
-    const float t = (float)(A.step0 + step + 1);
-    const float c1 = 1.f - powf(A.beta1, t), c2 = 1.f - powf(A.beta2, t);
-    for (int p = tid; p < NP; p += MLP_THREADS) {
-      const float gp = g[p];
-      const float mp = A.beta1 * A.m[p] + (1.f - A.beta1) * gp;
-      const float vp = A.beta2 * A.v[p] + (1.f - A.beta2) * gp * gp;
-      A.m[p] = mp;
-      A.v[p] = vp;
-      w[p] -= A.lr * (mp / c1) / (sqrtf(vp / c2) + A.eps);
-    }
-    __syncthreads();
-  }
-  for (int p = tid; p < NP; p += MLP_THREADS) A.params[p] = w[p];
+  // Adam (Kingma & Ba, 2015) with bias correction
+  const float t = (float) A.t;
+  const float c1 = 1.f - powf(A.beta1, t), c2 = 1.f - powf(A.beta2, t);
+  const float mp = A.beta1 * A.m[p] + (1.f - A.beta1) * g;
+  const float vp = A.beta2 * A.v[p] + (1.f - A.beta2) * g * g;
+  A.m[p] = mp;
+  A.v[p] = vp;
+  A.params[p] -= A.lr * (mp / c1) / (sqrtf(vp / c2) + A.eps);
 }
 
 } // namespace
@@ -220,7 +244,7 @@ struct tm_mlp {
   int device = 0;
   hipStream_t stream = nullptr;
   float *params = nullptr, *m = nullptr, *v = nullptr, *loss = nullptr;
-  float *x = nullptr, *y = nullptr, *act = nullptr, *out = nullptr;
+  float *x = nullptr, *y = nullptr, *part = nullptr, *lpart = nullptr, *out = nullptr;
   size_t cap = 0;
   uint32_t step = 0;
 };
@@ -236,12 +260,14 @@ namespace {
 int ensure(tm_mlp *m, size_t n)
 {
   if (n <= m->cap) return TM_OK;
-  hipFree(m->x); hipFree(m->y); hipFree(m->act); hipFree(m->out);
-  m->x = m->y = m->act = m->out = nullptr;
+  hipFree(m->x); hipFree(m->y); hipFree(m->part); hipFree(m->lpart); hipFree(m->out);
+  m->x = m->y = m->part = m->lpart = m->out = nullptr;
   m->cap = 0;
+  const size_t slices = (n + MLP_SLICE - 1) / MLP_SLICE;
   MLP_HIP(hipMalloc(&m->x, n * IN * sizeof(float)));
   MLP_HIP(hipMalloc(&m->y, n * sizeof(float)));
-  MLP_HIP(hipMalloc(&m->act, (n * 6 * H + n) * sizeof(float)));
+  MLP_HIP(hipMalloc(&m->part, slices * NP * sizeof(float)));
+  MLP_HIP(hipMalloc(&m->lpart, slices * sizeof(float)));
   MLP_HIP(hipMalloc(&m->out, n * sizeof(float)));
   m->cap = n;
   return TM_OK;
@@ -258,12 +284,20 @@ int run_train(tm_mlp *m, const float *X, const float *Y, uint32_t n, uint32_t st
   MLP_HIP(hipMemcpyAsync(m->y, Y, (size_t) n * 4, hipMemcpyHostToDevice, m->stream));
   float *gdev = nullptr;
   if (grad) MLP_HIP(hipMalloc(&gdev, NP * sizeof(float)));
-  mlp_train_args A;
-  A.params = m->params; A.m = m->m; A.v = m->v;
-  A.X = m->x; A.Y = m->y; A.act = m->act; A.grad_out = gdev; A.loss_out = m->loss;
-  A.n = n; A.steps = steps; A.step0 = m->step; A.mode = grad ? 1 : 0;
+  const uint32_t slices = (n + MLP_SLICE - 1) / MLP_SLICE;
+  mlp_grad_args G;
+  G.params = m->params; G.X = m->x; G.Y = m->y; G.part = m->part; G.lpart = m->lpart; G.n = n;
+  mlp_adam_args A;
+  A.params = m->params; A.m = m->m; A.v = m->v; A.part = m->part; A.lpart = m->lpart;
+  A.grad_out = gdev; A.loss_out = m->loss; A.slices = slices; A.n = n;
   A.lr = lr; A.beta1 = 0.9f; A.beta2 = 0.999f; A.eps = 1e-8f;
-  hipLaunchKernelGGL(mlp_train_kernel, dim3(1), dim3(MLP_THREADS), 0, m->stream, A);
+  const uint32_t nsteps = grad ? 1 : steps;
+  for (uint32_t step = 0; step < nsteps; step++) {
+    A.t = m->step + step + 1;
+    hipLaunchKernelGGL(mlp_grad_kernel, dim3(slices), dim3(MLP_THREADS), 0, m->stream, G);
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((NP + MLP_THREADS - 1) / MLP_THREADS), dim3(MLP_THREADS), 0, m->stream,
+                       A);
+  }
   MLP_HIP(hipGetLastError());
   if (grad) MLP_HIP(hipMemcpyAsync(grad, gdev, NP * sizeof(float), hipMemcpyDeviceToHost, m->stream));
   if (loss) MLP_HIP(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
@@ -345,7 +379,7 @@ void tm_mlp_destroy(tm_mlp *m)
   if (!m) return;
   if (m->stream) hipStreamSynchronize(m->stream);
   hipFree(m->params); hipFree(m->m); hipFree(m->v); hipFree(m->loss);
-  hipFree(m->x); hipFree(m->y); hipFree(m->act); hipFree(m->out);
+  hipFree(m->x); hipFree(m->y); hipFree(m->part); hipFree(m->lpart); hipFree(m->out);
   if (m->stream) hipStreamDestroy(m->stream);
   delete m;
 }

@@ -90,9 +90,14 @@ def build_sampler(spec, global_dev=None, rng=None):
 
         s = Sampler("range", spec.get("def"), sample, adapt)
         s.state = state
+        # the range's definition, for samplers of whole candidate pools
+        s.bounds = {"min": minv, "max": maxv, "log": is_log, "int": is_int, "pow2": is_pow2, "round": round_to,
+                    "shift": shift}
         return s
     if isinstance(spec, (list, tuple)) and len(spec) > 0:
-        return Sampler("list", spec[0], lambda c=None: spec[int(rng.integers(len(spec)))])
+        s = Sampler("list", spec[0], lambda c=None: spec[int(rng.integers(len(spec)))])
+        s.values = list(spec)
+        return s
     raise ValueError("Bad hyper-parameter specification: %r" % (spec,))
 
 

@@ -2,11 +2,12 @@
 
 An FP32 MLP (12 -> 3x64 ReLU -> 1, tm_mlp_* in include/tsetlin_amd.h, trained
 by the engine's own HIP forward/backward/Adam kernels) predicts a trial's dev
-accuracy from its hyper-parameters.  Each round the proposer fits it on every
-trial outcome so far, scores a large pool of candidates drawn from the search's
-own samplers, and proposes the best-scoring ones plus a share of plain draws.
-Plain random search stays the fallback (too little history, or a fit that is
-worse than predicting the mean on held-out trials) and the judge (the target
+accuracy from its hyper-parameters.  Each round the proposer fits an ensemble
+of them on every trial outcome so far, scores a pool of candidates drawn over
+the search's own space and refined around the best of them, and proposes a
+spread-out set of the best-scoring ones plus a share of plain draws.
+Plain random search stays the fallback (too little history, or fits that
+predict held-out trials worse than their mean) and the judge (the target
 accuracy comes from a plain search).  No reference counterpart: optimize.lua
 proposes from its samplers only (optimize.lua:391-516).
 """
@@ -112,44 +113,219 @@ class MLP:
 
 def effective_target(params):
     """The machine's target for these options (capi.c:480-483)."""
-    c = int(math.ceil(params["clauses"] / 8.0) * 8)
-    t = params.get("target", -1)
-    if t is None or t < 0:
-        return int(math.sqrt(c / 2.0) * params["clause_tolerance"])
-    if t >= 1:
-        return int(math.ceil(t))
-    return int(math.ceil(max(1.0, float(np.float32(c * t)))))
+    t = params.get("target")
+    return int(_effective_target(np.array([float(params["clauses"])]), np.array([-1.0 if t is None else float(t)]),
+                                 np.array([float(params["clause_tolerance"])]))[0])
+
+
+def _effective_target(clauses, target, tolerance):
+    """effective_target over arrays of options (clauses, target, tolerance)."""
+    c = np.ceil(clauses / 8.0) * 8
+    t = np.where(np.isnan(target), -1.0, target)
+    frac = np.ceil(np.maximum(1.0, (c * t).astype(np.float32).astype(np.float64)))
+    out = np.where(t < 0, np.floor(np.sqrt(c / 2.0) * tolerance), np.where(t >= 1, np.ceil(t), frac))
+    return out.astype(np.int64)
 
 
 def features(params, ctx):
     """The 12 surrogate inputs: log clauses, log specificity, target/clauses,
     log target, log tolerance, log maximum, include_bits, epochs, feature
     density, log classes, log features, best dev accuracy so far."""
-    c = max(8.0, float(params["clauses"]))
-    t = max(1, effective_target(params))
-    return [
-        math.log2(c) / 12.0,
-        math.log(max(1e-3, float(params["specificity"]))) / 4.0,
-        t / c,
-        math.log2(t) / 12.0,
-        math.log2(max(1.0, float(params.get("clause_tolerance", 1)))) / 7.0,
-        math.log2(max(1.0, float(params.get("clause_maximum", 1)))) / 7.0,
-        float(params.get("include_bits", 1)) / 4.0,
-        float(ctx.get("epochs", 10)) / 20.0,
-        float(ctx.get("density", 0.5)),
-        math.log(max(2.0, float(ctx.get("classes", 2)))) / 5.0,
-        math.log2(max(2.0, float(ctx.get("features", 2)))) / 16.0,
-        float(ctx.get("best", 0.0)),
-    ]
+    cols = {k: np.array([float(params.get(k, d) if params.get(k, d) is not None else d)])
+            for k, d in (("clauses", 8.0), ("specificity", 1.0), ("target", -1.0), ("clause_tolerance", 1.0),
+                         ("clause_maximum", 1.0), ("include_bits", 1.0))}
+    return [float(v) for v in features_batch(cols, ctx)[0]]
+
+
+def features_batch(cols, ctx):
+    """features() of a pool of candidates, one array per option: an (n, 12)
+    float32 matrix."""
+    c = np.maximum(8.0, cols["clauses"])
+    t = np.maximum(1, _effective_target(cols["clauses"], cols["target"], cols["clause_tolerance"])).astype(np.float64)
+    n = c.shape[0]
+    f = np.empty((n, INPUTS), np.float64)
+    f[:, 0] = np.log2(c) / 12.0
+    f[:, 1] = np.log(np.maximum(1e-3, cols["specificity"])) / 4.0
+    f[:, 2] = t / c
+    f[:, 3] = np.log2(t) / 12.0
+    f[:, 4] = np.log2(np.maximum(1.0, cols["clause_tolerance"])) / 7.0
+    f[:, 5] = np.log2(np.maximum(1.0, cols["clause_maximum"])) / 7.0
+    f[:, 6] = cols["include_bits"] / 4.0
+    f[:, 7] = float(ctx.get("epochs", 10)) / 20.0
+    f[:, 8] = float(ctx.get("density", 0.5))
+    f[:, 9] = math.log(max(2.0, float(ctx.get("classes", 2)))) / 5.0
+    f[:, 10] = math.log2(max(2.0, float(ctx.get("features", 2)))) / 16.0
+    f[:, 11] = float(ctx.get("best", 0.0))
+    return f.astype(np.float32)
+
+
+class Space:
+    """The search space of a set of samplers as arrays: candidates are drawn,
+    moved and decoded a whole pool at a time, with each range's bounds,
+    log scale and rounding as its sampler applies them (optimize.lua:187-258)."""
+
+    def __init__(self, samplers, names, rng):
+        self.rng = rng
+        self.ranges, self.lists, self.fixed = {}, {}, {}
+        for name in names:
+            s = samplers.get(name)
+            if s is None:
+                continue
+            if s.type == "range" and getattr(s, "bounds", None):
+                self.ranges[name] = (s, s.bounds)
+            elif s.type == "list" and getattr(s, "values", None):
+                self.lists[name] = s.values
+            else:
+                self.fixed[name] = s.sample()
+
+    @staticmethod
+    def _span(b):
+        lo, hi = b["min"] + b["shift"], b["max"] + b["shift"]
+        return (math.log(lo), math.log(hi) - math.log(lo)) if b["log"] else (b["min"], b["max"] - b["min"])
+
+    def _decode(self, b, x):
+        """Values (after the log map) to the sampler's rounded values."""
+        x = np.minimum(np.maximum(x, b["min"]), b["max"])
+        if b["pow2"]:
+            x = 2.0 ** np.floor(np.log(x) / math.log(2) + 0.5)
+        elif b["round"]:
+            x = np.floor(x / b["round"] + 0.5) * b["round"]
+        elif b["int"]:
+            x = np.floor(x + 0.5)
+        return np.minimum(np.maximum(x, b["min"]), b["max"])
+
+    def _from_unit(self, b, u):
+        base, span = self._span(b)
+        x = u * span + base
+        return self._decode(b, np.exp(x) - b["shift"] if b["log"] else x)
+
+    def _to_unit(self, b, v):
+        base, span = self._span(b)
+        x = np.log(v + b["shift"]) if b["log"] else v
+        return (x - base) / span if span > 0 else np.zeros_like(v)
+
+    def draw(self, n, centred):
+        """n candidates: uniform over each range (as a sampler without a
+        centre), or (centred) jittered around each sampler's centre with the
+        sampler's current jitter and reflection."""
+        cols = {}
+        for name, (s, b) in self.ranges.items():
+            if centred and s.center is not None:
+                c = math.log(s.center + b["shift"]) if b["log"] else float(s.center)
+                x = self.rng.normal(c, s.state["jitter"], n)
+                if b["log"]:
+                    x = np.exp(x) - b["shift"]
+                x = np.where(x > b["max"], 2 * b["max"] - x, x)
+                x = np.where(x < b["min"], 2 * b["min"] - x, x)
+                cols[name] = self._decode(b, x)
+            else:
+                cols[name] = self._from_unit(b, self.rng.random(n))
+        for name, vals in self.lists.items():
+            cols[name] = np.array(vals, np.float64)[self.rng.integers(len(vals), size=n)]
+        for name, v in self.fixed.items():
+            cols[name] = np.full(n, float(v))
+        return cols
+
+    def move(self, cols, idx, per, scales=(0.03, 0.1, 0.25), snap=0.15, redraw=0.25):
+        """`per` local moves from each candidate in idx: every range coordinate
+        moves by a normal step (one of `scales`, in units of the range) and
+        reflects at the bounds, or, with probability `snap`, goes to the
+        nearer bound; a list option is drawn again with probability `redraw`."""
+        src = np.repeat(np.asarray(idx), per)
+        m = src.shape[0]
+        out = {}
+        sc = np.array(scales)[self.rng.integers(len(scales), size=m)]
+        for name, (s, b) in self.ranges.items():
+            u = self._to_unit(b, cols[name][src])
+            u = u + sc * self.rng.standard_normal(m)
+            u = np.where(u > 1, 2 - u, u)
+            u = np.where(u < 0, -u, u)
+            u = np.clip(u, 0.0, 1.0)
+            snapped = self.rng.random(m) < snap
+            u = np.where(snapped, np.round(u), u)
+            out[name] = self._from_unit(b, u)
+        for name, vals in self.lists.items():
+            again = self.rng.random(m) < redraw
+            out[name] = np.where(again, np.array(vals, np.float64)[self.rng.integers(len(vals), size=m)],
+                                 cols[name][src])
+        for name, v in self.fixed.items():
+            out[name] = np.full(m, float(v))
+        return out
+
+    def unit(self, cols):
+        """Each candidate's coordinates in the unit box (log ranges on their
+        log scale; a list option as its index over the list's length)."""
+        parts = [self._to_unit(b, cols[name]) for name, (s, b) in self.ranges.items()]
+        for name, vals in self.lists.items():
+            v = np.array(vals, np.float64)
+            parts.append(np.argmin(np.abs(cols[name][:, None] - v[None, :]), 1) / max(1, len(vals) - 1))
+        n = next(iter(cols.values())).shape[0]
+        return np.stack(parts, 1) if parts else np.zeros((n, 0))
+
+    def params(self, cols, i):
+        """Candidate i as an options dict (integral ranges as ints)."""
+        p = {}
+        for name, (s, b) in self.ranges.items():
+            v = float(cols[name][i])
+            p[name] = int(v) if (b["int"] or b["round"]) and float(v).is_integer() else v
+        for name in self.lists:
+            v = float(cols[name][i])
+            p[name] = next((x for x in self.lists[name] if float(x) == v), v)
+        for name, v in self.fixed.items():
+            p[name] = v
+        return p
+
+
+def _swapped(cols):
+    """optimize._swap_tolerance over a pool: tolerance and maximum swapped
+    where the tolerance is the larger."""
+    out = dict(cols)
+    if "clause_tolerance" in cols and "clause_maximum" in cols:
+        a, b = cols["clause_tolerance"], cols["clause_maximum"]
+        sw = (a > 0) & (b > 0) & (a > b)
+        out["clause_tolerance"], out["clause_maximum"] = np.where(sw, b, a), np.where(sw, a, b)
+    return out
+
+
+def _with_defaults(cols, n):
+    full = dict(cols)
+    for k, d in (("clauses", 8.0), ("specificity", 1.0), ("target", -1.0), ("clause_tolerance", 1.0),
+                 ("clause_maximum", 1.0), ("include_bits", 1.0)):
+        if k not in full:
+            full[k] = np.full(n, d)
+    return full
+
+
+def _concat(a, b):
+    return {k: np.concatenate([a[k], b[k]]) for k in a}
 
 
 class Proposer:
-    """propose(samplers, n, history) for optimize.search_batched."""
+    """propose(samplers, n, history) for optimize.search_batched.
+
+    The first round is a share `initial` of draws spread uniformly over the
+    whole space, the rest plain draws, so the first fit sees more than the
+    search's starting neighbourhood.  From `min_history` trials on,
+    `ensemble` networks are fitted to every trial so far (the first on all of
+    them, the others on bootstrap resamples), and candidates are scored by the
+    ensemble's mean prediction less `kappa` times its spread, so settings where
+    the fits disagree (far from the trials seen) are discounted.  The pool:
+    half drawn around the search's sampler centres, half uniform over the
+    space; then `refine` times, `pool` local moves from the `refine_top`
+    best-scoring candidates (a share `snap` of the coordinates pushed onto the
+    nearer bound, where the best settings of a space often lie) are scored and
+    added.  The network's picks are the best-scoring candidates not tried
+    before that keep a distance `spread` (unit box) from each other, and a
+    share `explore` of the round stays plain draws."""
 
     def __init__(self, ctx, param_names=opt.PARAM_NAMES, seed=0, pool=4096, explore=0.25, min_history=48,
-                 steps=300, lr=3e-3, key=opt.make_key):
+                 steps=300, lr=3e-3, key=opt.make_key, refine=2, refine_top=64, ensemble=4, kappa=0.5, snap=0.15,
+                 spread=0.2, initial=0.5, folds=5):
         self.ctx, self.names, self.pool, self.explore = dict(ctx), param_names, pool, explore
         self.min_history, self.steps, self.lr, self.key = min_history, steps, lr, key
+        self.refine, self.refine_top = refine, refine_top
+        self.ensemble, self.kappa, self.snap, self.spread = max(1, ensemble), kappa, snap, spread
+        self.initial, self.folds = initial, max(2, folds)
         self.rng = np.random.default_rng(seed)
         self.seed = seed
         self.last = {"mode": "plain"}
@@ -161,55 +337,90 @@ class Proposer:
         y = np.array([s for _, s in history], np.float32)
         mu, sd = float(y.mean()), float(y.std() + 1e-6)
         Y = (y - mu) / sd
-        # held-out check: fit on 80%, score the rest
+        # held-out check: out-of-fold predictions of `folds` fits, each
+        # against the mean of its own training part
         idx = self.rng.permutation(len(y))
-        cut = max(1, int(0.8 * len(y)))
-        net = MLP(seed=self.seed + len(y))
-        net.train(X[idx[:cut]], Y[idx[:cut]], self.steps, self.lr)
-        pred = net.predict(X[idx[cut:]])
-        resid = ((pred - Y[idx[cut:]]) ** 2).sum()
-        total = ((Y[idx[cut:]] - Y[idx[:cut]].mean()) ** 2).sum() + 1e-9
-        r2 = 1.0 - resid / total
-        net.destroy()
-        net = MLP(seed=self.seed + len(y))
-        loss = net.train(X, Y, self.steps, self.lr)
-        return net, ctx, r2, loss
+        resid = total = 0.0
+        for f in range(self.folds):
+            test = idx[f::self.folds]
+            train = np.setdiff1d(idx, test)
+            if test.size == 0 or train.size == 0:
+                continue
+            net = MLP(seed=self.seed + len(y) + f)
+            net.train(X[train], Y[train], self.steps, self.lr)
+            resid += float(((net.predict(X[test]) - Y[test]) ** 2).sum())
+            total += float(((Y[test] - Y[train].mean()) ** 2).sum())
+            net.destroy()
+        r2 = 1.0 - resid / (total + 1e-9)
+        nets, loss = [], None
+        for e in range(self.ensemble):
+            net = MLP(seed=self.seed + len(y) + 7919 * e)
+            sel = np.arange(len(y)) if e == 0 else self.rng.integers(len(y), size=len(y))
+            l = net.train(X[sel], Y[sel], self.steps, self.lr)
+            loss = l if loss is None else loss
+            nets.append(net)
+        return nets, ctx, r2, loss
 
     def __call__(self, samplers, n, history):
         plain = [opt.sample_params(samplers, self.names) for _ in range(n)]
         if len(history) < self.min_history:
             self.last = {"mode": "plain", "reason": "history %d < %d" % (len(history), self.min_history)}
+            if not history and self.initial > 0:
+                # the first round: a share `initial` drawn uniformly over the
+                # whole space, so the first fit sees more than the search's
+                # starting neighbourhood
+                space = Space(samplers, self.names, self.rng)
+                k = int(round(self.initial * n))
+                cols = space.draw(k, False)
+                self.last["initial"] = k
+                return [space.params(cols, i) for i in range(k)] + plain[: n - k]
             return plain
-        net, ctx, r2, loss = self._fit(history)
+        nets, ctx, r2, loss = self._fit(history)
         if not r2 > 0.0:
-            net.destroy()
+            for net in nets:
+                net.destroy()
             self.last = {"mode": "plain", "reason": "held-out R2 %.3f" % r2}
             return plain
-        # candidate pool: draws around the search centre plus global draws
-        cand = [opt.sample_params(samplers, self.names) for _ in range(self.pool // 2)]
-        for _ in range(self.pool - len(cand)):
-            p = {}
-            for name in self.names:
-                s = samplers.get(name)
-                if s is not None:
-                    p[name] = s.sample(None) if s.type == "range" else s.sample()
-            cand.append(p)
-        X = np.array([features(opt._swap_tolerance(dict(p)), ctx) for p in cand], np.float32)
-        score = net.predict(X)
-        net.destroy()
+        space = Space(samplers, self.names, self.rng)
+
+        def score(cols):
+            m = next(iter(cols.values())).shape[0]
+            F = features_batch(_with_defaults(_swapped(cols), m), ctx)
+            P = np.stack([net.predict(F) for net in nets])
+            return P.mean(0) - (self.kappa * P.std(0) if len(nets) > 1 else 0.0)
+
+        half = self.pool // 2
+        cols = _concat(space.draw(half, True), space.draw(self.pool - half, False))
+        sc = score(cols)
+        for _ in range(self.refine):
+            top = np.argsort(-sc)[: self.refine_top]
+            moved = space.move(cols, top, max(1, self.pool // max(1, len(top))), snap=self.snap)
+            cols = _concat(cols, moved)
+            sc = np.concatenate([sc, score(moved)])
+        for net in nets:
+            net.destroy()
         seen = {self.key(p) for p, _ in history}
         n_model = n - int(round(self.explore * n))
         picks = []
-        for i in np.argsort(-score):
-            k = self.key(cand[i])
+        U = space.unit(cols) if self.spread > 0 else None
+        chosen = []
+        for i in np.argsort(-sc, kind="stable"):
+            # a pick keeps at least `spread` (Euclidean, unit box) from the
+            # round's other picks, so the round covers more than one optimum
+            if U is not None and chosen and np.min(np.sqrt(((U[chosen] - U[i]) ** 2).sum(1))) < self.spread:
+                continue
+            p = space.params(cols, i)
+            k = self.key(opt._swap_tolerance(dict(p)))
             if k in seen:
                 continue
             seen.add(k)
-            picks.append(cand[i])
+            picks.append(p)
+            chosen.append(i)
             if len(picks) >= n_model:
                 break
         n_picked = len(picks)
         picks += plain[: n - len(picks)]
         # the first n_model proposals are the network's picks, the rest plain draws
-        self.last = {"mode": "surrogate", "r2": float(r2), "loss": float(loss), "n_model": n_picked}
+        self.last = {"mode": "surrogate", "r2": float(r2), "loss": float(loss), "n_model": n_picked,
+                     "pool": int(sc.shape[0])}
         return picks

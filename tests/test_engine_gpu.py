@@ -223,7 +223,11 @@ def test_step_queue_matches_scan(features, classes, clauses, neg, n, monkeypatch
     assert np.array_equal(got["1"][0], orc.actions)
     mask = count_mask(orc)
     assert np.array_equal(got["1"][3] & mask, orc.counts & mask)
-    assert got["1"][1]["steps"] == int(orc.stats[0])
+    # the event counters (kept in LDS on the four-workgroup tile) against the
+    # oracle's: steps, Type I true, Type II, Type I false, draws
+    st = got["1"][1]
+    assert (st["steps"], st["type1_true"], st["type2_true"], st["type1_false"], st["draws"]) == \
+        tuple(int(v) for v in orc.stats[:5])
 
 
 @pytest.mark.parametrize("tolerance", [8191, 8192, 9000])

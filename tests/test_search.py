@@ -198,3 +198,40 @@ def test_surrogate_features_and_target():
     assert S.effective_target(dict(p, target=-1)) == int(np.sqrt(1024) * 64)
     f = S.features(p, {"epochs": 10, "density": 0.05, "classes": 20, "features": 4096, "best": 0.9})
     assert len(f) == 12 and f[2] == pytest.approx(0.125) and f[0] == pytest.approx(11 / 12)
+
+
+SPACE = {"clauses": {"def": 512, "min": 64, "max": 4096, "log": True, "round": 8},
+         "clause_tolerance": {"def": 8, "min": 1, "max": 64, "log": True, "int": True},
+         "clause_maximum": {"def": 16, "min": 1, "max": 64, "log": True, "int": True},
+         "target": [0.125, 0.25],
+         "specificity": {"def": 10, "min": 2, "max": 40},
+         "include_bits": 1}
+
+
+def test_candidate_space_matches_samplers():
+    """The proposer's pooled candidates: every draw and local move stays on
+    the sampler's grid (bounds, rounding, list values); pooled features equal
+    features() of each candidate after the tolerance swap; the centred pool
+    follows the sampler's own distribution; moves reach the bounds."""
+    from santoku.tsetlin import surrogate as S
+    smp = opt.build_samplers(SPACE, opt.PARAM_NAMES, 0.3, np.random.default_rng(3))
+    sp = S.Space(smp, opt.PARAM_NAMES, np.random.default_rng(1))
+    ctx = {"epochs": 10, "density": 0.5, "classes": 4, "features": 256, "best": 0.6}
+    cols = S._concat(sp.draw(2000, True), sp.draw(2000, False))
+    moved = sp.move(cols, np.arange(64), 64)
+    pool = S._concat(cols, moved)
+    n = pool["clauses"].shape[0]
+    F = S.features_batch(S._with_defaults(S._swapped(pool), n), ctx)
+    for i in range(0, n, 5):
+        p = sp.params(pool, i)
+        assert isinstance(p["clauses"], int) and 64 <= p["clauses"] <= 4096 and p["clauses"] % 8 == 0, p
+        assert isinstance(p["clause_tolerance"], int) and 1 <= p["clause_tolerance"] <= 64, p
+        assert 2 <= p["specificity"] <= 40 and p["target"] in (0.125, 0.25) and p["include_bits"] == 1, p
+        want = np.array(S.features(opt._swap_tolerance(dict(p)), ctx), np.float32)
+        assert np.array_equal(want, F[i]), (p, want, F[i])
+    ref = [opt.sample_params(smp, opt.PARAM_NAMES) for _ in range(4000)]
+    for name in ("clauses", "specificity", "clause_tolerance"):
+        a, b = np.array([r[name] for r in ref], float), cols[name][:2000]
+        assert abs(np.median(a) - np.median(b)) <= 0.1 * np.median(a) + 1, name
+    assert (moved["clauses"] == 4096).any() and (moved["specificity"] == 40).any()
+    assert (moved["clause_tolerance"] == 1).any() and (moved["clauses"] == 64).any()
