@@ -401,6 +401,9 @@ void tm_batch_destroy(tm_batch *b);
 #define TM_MLP_PARAMS 9217
 typedef struct tm_mlp tm_mlp;
 int tm_mlp_create(const float *params, tm_mlp **out);
+/* Start again from `params` with zeroed Adam moments and step count, keeping
+ * the network's stream and buffers (a search refits every round). */
+int tm_mlp_reset(tm_mlp *m, const float *params);
 /* `steps` Adam steps (beta 0.9/0.999, eps 1e-8) on n samples X[n][12] -> Y[n];
  * *loss receives the mean squared error of the last step (may be NULL). */
 int tm_mlp_train(tm_mlp *m, const float *X, const float *Y, uint32_t n, uint32_t steps, float lr, float *loss);

@@ -336,6 +336,18 @@ int tm_mlp_create(const float *params, tm_mlp **out)
   return TM_OK;
 }
 
+int tm_mlp_reset(tm_mlp *m, const float *params)
+{
+  if (!m || !params) return mlp_fail(TM_E_ARG, "mlp reset: NULL argument");
+  MLP_HIP(hipSetDevice(m->device));
+  MLP_HIP(hipMemcpyAsync(m->params, params, NP * 4, hipMemcpyHostToDevice, m->stream));
+  MLP_HIP(hipMemsetAsync(m->m, 0, NP * 4, m->stream));
+  MLP_HIP(hipMemsetAsync(m->v, 0, NP * 4, m->stream));
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  m->step = 0;
+  return TM_OK;
+}
+
 int tm_mlp_train(tm_mlp *m, const float *X, const float *Y, uint32_t n, uint32_t steps, float lr, float *loss)
 {
   return run_train(m, X, Y, n, steps, lr, nullptr, loss);

@@ -40,6 +40,7 @@ def _lib():
         vp, u32 = ctypes.c_void_p, ctypes.c_uint32
         sigs = {
             "tm_mlp_create": (ctypes.c_int, [fp, ctypes.POINTER(vp)]),
+            "tm_mlp_reset": (ctypes.c_int, [vp, fp]),
             "tm_mlp_train": (ctypes.c_int, [vp, fp, fp, u32, u32, ctypes.c_float, fp]),
             "tm_mlp_gradient": (ctypes.c_int, [vp, fp, fp, u32, fp, fp]),
             "tm_mlp_predict": (ctypes.c_int, [vp, fp, u32, fp]),
@@ -72,6 +73,12 @@ class MLP:
     def _check(self, rc):
         if rc != 0:
             raise TsetlinError("%s: %s" % (_c.ERRORS.get(rc, rc), self._L.tm_mlp_last_error().decode()))
+
+    def reset(self, params=None, seed=0):
+        """Start again from `params` (or a fresh He-normal init from `seed`)
+        with zeroed Adam state, on the same device buffers."""
+        p, pp = _f32(init_params(seed) if params is None else params)
+        self._check(self._L.tm_mlp_reset(self._h, pp))
 
     def train(self, X, Y, steps, lr=3e-3):
         x, xp = _f32(X)
@@ -329,6 +336,19 @@ class Proposer:
         self.rng = np.random.default_rng(seed)
         self.seed = seed
         self.last = {"mode": "plain"}
+        self._nets = []
+
+    def _net(self, i, seed):
+        """Network i of the proposer's pool, started from a fresh init."""
+        while len(self._nets) <= i:
+            self._nets.append(MLP(seed=seed))
+        self._nets[i].reset(seed=seed)
+        return self._nets[i]
+
+    def close(self):
+        for net in self._nets:
+            net.destroy()
+        self._nets = []
 
     def _fit(self, history):
         best = max(s for _, s in history)
@@ -346,15 +366,14 @@ class Proposer:
             train = np.setdiff1d(idx, test)
             if test.size == 0 or train.size == 0:
                 continue
-            net = MLP(seed=self.seed + len(y) + f)
+            net = self._net(0, self.seed + len(y) + f)
             net.train(X[train], Y[train], self.steps, self.lr)
             resid += float(((net.predict(X[test]) - Y[test]) ** 2).sum())
             total += float(((Y[test] - Y[train].mean()) ** 2).sum())
-            net.destroy()
         r2 = 1.0 - resid / (total + 1e-9)
         nets, loss = [], None
         for e in range(self.ensemble):
-            net = MLP(seed=self.seed + len(y) + 7919 * e)
+            net = self._net(e, self.seed + len(y) + 7919 * e)
             sel = np.arange(len(y)) if e == 0 else self.rng.integers(len(y), size=len(y))
             l = net.train(X[sel], Y[sel], self.steps, self.lr)
             loss = l if loss is None else loss
@@ -377,8 +396,6 @@ class Proposer:
             return plain
         nets, ctx, r2, loss = self._fit(history)
         if not r2 > 0.0:
-            for net in nets:
-                net.destroy()
             self.last = {"mode": "plain", "reason": "held-out R2 %.3f" % r2}
             return plain
         space = Space(samplers, self.names, self.rng)
@@ -397,8 +414,6 @@ class Proposer:
             moved = space.move(cols, top, max(1, self.pool // max(1, len(top))), snap=self.snap)
             cols = _concat(cols, moved)
             sc = np.concatenate([sc, score(moved)])
-        for net in nets:
-            net.destroy()
         seen = {self.key(p) for p, _ in history}
         n_model = n - int(round(self.explore * n))
         picks = []

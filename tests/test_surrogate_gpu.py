@@ -50,6 +50,21 @@ def test_adam_steps_match_oracle_and_reduce_loss():
     assert MO.loss(net.params(), X, Y) < 0.2 * l0
 
 
+def test_reset_equals_a_fresh_network():
+    """tm_mlp_reset: parameters, Adam moments and step count start again, so
+    training after a reset equals training a new network (the slice sums
+    are in a fixed order, so equal means bit-equal)."""
+    X, Y = data(4, 300)
+    p = S.init_params(9)
+    a = S.MLP(p)
+    a.train(X, Y, 20, 1e-3)
+    a.reset(p)
+    a.train(X, Y, 20, 1e-3)
+    b = S.MLP(p)
+    b.train(X, Y, 20, 1e-3)
+    assert np.array_equal(a.params(), b.params())
+
+
 SPACE = {"clauses": {"def": 512, "min": 64, "max": 4096, "log": True, "round": 8},
          "clause_tolerance": {"def": 8, "min": 1, "max": 64, "log": True, "int": True},
          "clause_maximum": {"def": 16, "min": 1, "max": 64, "log": True, "int": True},
