@@ -7,11 +7,13 @@ clause tolerance = maximum = 64, state 8 bits.  A step is one training epoch
 over the 200k samples (inputs resident in HBM).  `value` is training
 samples/s; predict samples/s over the 20k test rows is reported beside it.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-search]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-search] [--no-encoder]
 
 The same line carries workloads 4 and 5 (batched search trials, surrogate-
 guided vs plain search on workload 1's data) under "search_batched_trials"
-and "search_surrogate_vs_plain".
+and "search_surrogate_vs_plain", and workload 3 (the encoder at its full
+machine shape: one epoch over 16,384 samples, predict over 500,000) under
+"encoder"; `--encoder` prints the encoder's line alone.
 
 The roofline object prices the train kernel against the integer VALU peak
 of gfx950: 256 CUs x 4 SIMDs x 32 lanes x 2.4 GHz = 78.6 T lane-ops/s for
@@ -278,7 +280,7 @@ def encoder_bench(args):
             "speedup_train": round((n_tr / wall) / cb["train_sps"], 1),
             "speedup_predict": round((n_pr / pwall) / cb["predict_sps"], 1),
         }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 SEARCH_SPACE = {
@@ -448,7 +450,8 @@ def main():
     ap.add_argument("--no-search", action="store_true", help="skip workloads 4 and 5 (batched trials, surrogate search)")
     ap.add_argument("--search-seeds", type=int, default=8)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--encoder", action="store_true", help="workload 3: the 256-bit encoder at full machine shape")
+    ap.add_argument("--encoder", action="store_true", help="workload 3 only: the 256-bit encoder at full machine shape")
+    ap.add_argument("--no-encoder", action="store_true", help="leave workload 3 out of the default line")
     ap.add_argument("--enc-train", type=int, default=16_384)
     ap.add_argument("--enc-predict", type=int, default=500_000)
     ap.add_argument("--data", help=argparse.SUPPRESS)
@@ -479,7 +482,7 @@ def main():
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.encoder:
-        encoder_bench(args)
+        print(json.dumps(encoder_bench(args)), flush=True)
         return
     if args.copy_calibration:
         import torch
@@ -605,6 +608,14 @@ def main():
         }
     if not args.no_search:
         line["search_batched_trials"], line["search_surrogate_vs_plain"] = search_measure(args.seed, args.search_seeds)
+    if not args.no_encoder:
+        # workload 3 (the encoder at its full machine shape), as bench.py --encoder
+        del eng
+        torch.cuda.empty_cache()
+        enc = encoder_bench(args)
+        line["encoder"] = {k: enc[k] for k in ("config", "value", "unit", "train_kernel_ms", "predict_samples_per_s",
+                                               "predict_kernel_ms", "held_out_bit_error_after_epoch", "roofline",
+                                               "cpu_baseline")}
     print(json.dumps(line), flush=True)
 
 
