@@ -180,10 +180,18 @@ class Space:
                 continue
             if s.type == "range" and getattr(s, "bounds", None):
                 self.ranges[name] = (s, s.bounds)
-            elif s.type == "list" and getattr(s, "values", None):
+            elif s.type == "list" and getattr(s, "values", None) and \
+                    all(isinstance(v, (int, float)) and not isinstance(v, bool) for v in s.values):
                 self.lists[name] = s.values
             else:
+                # a fixed option, or one the pool cannot vary numerically:
+                # held at one draw for the whole pool
                 self.fixed[name] = s.sample()
+
+    def _numeric_fixed(self):
+        """The fixed options that enter the pool's arrays (others only
+        reach the proposed options)."""
+        return [(k, v) for k, v in self.fixed.items() if isinstance(v, (int, float)) and not isinstance(v, bool)]
 
     @staticmethod
     def _span(b):
@@ -229,7 +237,7 @@ class Space:
                 cols[name] = self._from_unit(b, self.rng.random(n))
         for name, vals in self.lists.items():
             cols[name] = np.array(vals, np.float64)[self.rng.integers(len(vals), size=n)]
-        for name, v in self.fixed.items():
+        for name, v in self._numeric_fixed():
             cols[name] = np.full(n, float(v))
         return cols
 
@@ -255,7 +263,7 @@ class Space:
             again = self.rng.random(m) < redraw
             out[name] = np.where(again, np.array(vals, np.float64)[self.rng.integers(len(vals), size=m)],
                                  cols[name][src])
-        for name, v in self.fixed.items():
+        for name, v in self._numeric_fixed():
             out[name] = np.full(m, float(v))
         return out
 

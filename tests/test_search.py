@@ -235,3 +235,18 @@ def test_candidate_space_matches_samplers():
         assert abs(np.median(a) - np.median(b)) <= 0.1 * np.median(a) + 1, name
     assert (moved["clauses"] == 4096).any() and (moved["specificity"] == 40).any()
     assert (moved["clause_tolerance"] == 1).any() and (moved["clauses"] == 64).any()
+
+
+def test_candidate_space_keeps_non_numeric_options_fixed():
+    """An option the pool cannot vary numerically (a list of strings, a
+    boolean) is held at one draw and passed through to the proposals."""
+    from santoku.tsetlin import surrogate as S
+    space = {"clauses": {"def": 64, "min": 8, "max": 512, "log": True, "round": 8}, "target": [0.25, 0.5],
+             "specificity": 4.0, "clause_tolerance": ["a", "b"], "include_bits": True}
+    smp = opt.build_samplers(space, opt.PARAM_NAMES, 0.3, np.random.default_rng(0))
+    sp = S.Space(smp, opt.PARAM_NAMES, np.random.default_rng(0))
+    cols = sp.draw(8, False)
+    moved = sp.move(cols, [0, 1], 4)
+    for p in [sp.params(cols, i) for i in range(8)] + [sp.params(moved, i) for i in range(8)]:
+        assert p["clause_tolerance"] in ("a", "b") and p["include_bits"] is True and p["specificity"] == 4.0
+        assert 8 <= p["clauses"] <= 512 and p["target"] in (0.25, 0.5)
