@@ -81,7 +81,17 @@ def profiled_traffic():
         return None
 
 
-def cpu_baseline(data, threads, budget_s=15.0):
+def _grow_sample(run, limit, cap, target_s):
+    """Grow a CPU sample until one timed run takes at least half of
+    `target_s` (so the timed run lands near 10-20 s of CPU work)."""
+    t = run(limit)
+    while t < target_s / 2 and limit < cap:
+        limit = min(cap, int(limit * min(8.0, max(1.5, target_s / max(t, 1e-3)))))
+        t = run(limit)
+    return limit, t
+
+
+def cpu_baseline(data, threads, target_s=12.0):
     """Reference-shaped CPU training on a bounded slice: one epoch over the
     first `limit` samples of each thread's shuffle, all chunks."""
     import numpy as np
@@ -102,11 +112,7 @@ def cpu_baseline(data, threads, budget_s=15.0):
         return time.perf_counter() - t0
 
     run(4)
-    limit = 16
-    t = run(limit)
-    while t < budget_s / 4 and limit < 20000:
-        limit = int(limit * min(8.0, max(2.0, budget_s / 2 / max(t, 1e-3))))
-        t = run(limit)
+    limit, t = _grow_sample(run, 16, 20000, target_s)
     train_sps = limit / t
     # predict: bounded test slice on the same machine size
     m = O.Machine(p)
@@ -121,7 +127,7 @@ def cpu_baseline(data, threads, budget_s=15.0):
         L.orc_predict_classifier_threads(O.ctypes.byref(p), O.ptr(m.actions), O.ptr(test_rows[:npred]),
                                          npred, O.ptr(out, O._I64P), threads)
         tp = time.perf_counter() - t0
-        if tp >= 2.0 or npred >= test_rows.shape[0]:
+        if tp >= 5.0 or npred >= test_rows.shape[0]:
             break
         npred = min(npred * 4, test_rows.shape[0])
     return dict(train_sps=train_sps, train_sample=limit, train_s=t, predict_sps=npred / tp, predict_sample=npred,
@@ -155,7 +161,7 @@ def encoder_rows_gpu(n, seed, dev, block=25_000):
     return rows, codes
 
 
-def cpu_baseline_encoder(rows, codes, threads, budget_s=15.0):
+def cpu_baseline_encoder(rows, codes, threads, target_s=12.0):
     """Reference-shaped CPU encoder training on a bounded slice (one epoch over
     the first `limit` samples of each thread's shuffle, all 32,768 chunks)
     and predict over a bounded slice."""
@@ -170,18 +176,18 @@ def cpu_baseline_encoder(rows, codes, threads, budget_s=15.0):
     p = O.params("encoder", **{("classes" if k == "hidden" else "features" if k == "visible" else k): v
                                for k, v in ENCODER.items()})
 
+    last = {}
+
     def run(limit):
         m = O.Machine(p)
         t0 = time.perf_counter()
         m.train_refshape(rows, codes=codes, epochs=1, threads=threads, sample_limit=limit, L=L)
-        return time.perf_counter() - t0, m
+        last["m"] = m
+        return time.perf_counter() - t0
 
     run(1)
-    limit = 2
-    t, m = run(limit)
-    while t < budget_s / 4 and limit < rows.shape[0]:
-        limit = min(rows.shape[0], int(limit * min(8.0, max(2.0, budget_s / 2 / max(t, 1e-3)))))
-        t, m = run(limit)
+    limit, t = _grow_sample(run, 2, rows.shape[0], target_s)
+    m = last["m"]       # the machine of the timed run predicts below
     npred = 4
     while True:
         out = np.zeros((npred, 32), np.uint8)
@@ -189,7 +195,7 @@ def cpu_baseline_encoder(rows, codes, threads, budget_s=15.0):
         L.orc_predict_encoder_threads(O.ctypes.byref(p), O.ptr(m.actions), O.ptr(rows[:npred]), npred,
                                       O.ptr(out), threads)
         tp = time.perf_counter() - t0
-        if tp >= 2.0 or npred >= rows.shape[0]:
+        if tp >= 5.0 or npred >= rows.shape[0]:
             break
         npred = min(npred * 4, rows.shape[0])
     return dict(train_sps=limit / t, train_sample=limit, train_s=t, predict_sps=npred / tp, predict_sample=npred,
