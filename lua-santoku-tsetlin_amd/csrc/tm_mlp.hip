@@ -16,6 +16,7 @@
 #include <string.h>
 #include <vector>
 
+
 namespace {
 
 constexpr int IN = TM_MLP_INPUTS, H = TM_MLP_HIDDEN;
@@ -101,15 +102,16 @@ struct mlp_grad_args {
 };
 
 // out[s][i] = relu(b[i] + sum_k W[i][k] in[s][k]) with the weights transposed
-// (wt[k][i]): the lanes of a wave take consecutive units i of one sample, so
-// the weight reads are consecutive and the input read is a broadcast
+// (wt[k][i], rows of HP floats so that the transposing writes spread over the
+// banks): the lanes of a wave take consecutive units i of one sample, so the
+// weight reads are consecutive and the input read is a broadcast
 __device__ __forceinline__ void fwd_layer(const float *wt, const float *b, int nin, const float *in, int in_stride,
                                           float *out, int tid)
 {
   for (int idx = tid; idx < MLP_SLICE * H; idx += MLP_THREADS) {
     const int s = idx / H, i = idx % H;
     float a = b[i];
-    for (int k = 0; k < nin; k++) a = fmaf(wt[k * H + i], in[s * in_stride + k], a);
+    for (int k = 0; k < nin; k++) a = fmaf(wt[k * HP + i], in[s * in_stride + k], a);
     out[s * HP + i] = fmaxf(a, 0.f);
   }
 }
@@ -128,7 +130,7 @@ __device__ __forceinline__ void back_layer(const float *w, const float *dn, cons
 __global__ __launch_bounds__(MLP_THREADS) void mlp_grad_kernel(mlp_grad_args A)
 {
   __shared__ float w[NP];
-  __shared__ float wt[IN * H + 2 * H * H];          // W1, W2, W3 transposed
+  __shared__ float wt[IN * HP + 2 * H * HP];        // W1, W2, W3 transposed, rows of HP floats
   __shared__ float xs[MLP_SLICE * IN];
   __shared__ float act[3][MLP_SLICE * HP];          // post-ReLU activations of the three hidden layers
   __shared__ float del[3][MLP_SLICE * HP];          // their deltas
@@ -136,22 +138,42 @@ __global__ __launch_bounds__(MLP_THREADS) void mlp_grad_kernel(mlp_grad_args A)
   const int tid = threadIdx.x;
   const uint32_t s0 = blockIdx.x * MLP_SLICE;
   const uint32_t ns = min((uint32_t) MLP_SLICE, A.n - s0);
-  for (int p = tid; p < NP; p += MLP_THREADS) {
-    const float v = A.params[p];
+  // the parameters as 16-byte loads, all of a thread's in flight together
+  // (one at a time, each waiting for the last, they took most of a step)
+  auto put = [&](int p, float v) {
     w[p] = v;
-    if (p < OB1) wt[(p % IN) * H + p / IN] = v;
-    else if (p >= OW2 && p < OB2) wt[IN * H + ((p - OW2) % H) * H + (p - OW2) / H] = v;
-    else if (p >= OW3 && p < OB3) wt[IN * H + H * H + ((p - OW3) % H) * H + (p - OW3) / H] = v;
+    if (p < OB1) wt[(p % IN) * HP + p / IN] = v;
+    else if (p >= OW2 && p < OB2) wt[IN * HP + ((p - OW2) % H) * HP + (p - OW2) / H] = v;
+    else if (p >= OW3 && p < OB3) wt[IN * HP + H * HP + ((p - OW3) % H) * HP + (p - OW3) / H] = v;
+  };
+  constexpr int NQ = NP / 4, QPT = (NQ + MLP_THREADS - 1) / MLP_THREADS;
+  const float4 *pq = reinterpret_cast<const float4 *>(A.params);
+  float4 q[QPT];
+#pragma unroll
+  for (int u = 0; u < QPT; u++) {
+    const int qi = tid + u * MLP_THREADS;
+    q[u] = qi < NQ ? pq[qi] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
+#pragma unroll
+  for (int u = 0; u < QPT; u++) {
+    const int qi = tid + u * MLP_THREADS;
+    if (qi < NQ) {
+      put(4 * qi, q[u].x);
+      put(4 * qi + 1, q[u].y);
+      put(4 * qi + 2, q[u].z);
+      put(4 * qi + 3, q[u].w);
+    }
+  }
+  for (int p = 4 * NQ + tid; p < NP; p += MLP_THREADS) put(p, A.params[p]);
   // samples past the slice's end enter as zeros and leave with zero deltas
   for (int idx = tid; idx < MLP_SLICE * IN; idx += MLP_THREADS)
     xs[idx] = (uint32_t)(idx / IN) < ns ? A.X[(size_t) s0 * IN + idx] : 0.f;
   __syncthreads();
   fwd_layer(wt, w + OB1, IN, xs, IN, act[0], tid);
   __syncthreads();
-  fwd_layer(wt + IN * H, w + OB2, H, act[0], HP, act[1], tid);
+  fwd_layer(wt + IN * HP, w + OB2, H, act[0], HP, act[1], tid);
   __syncthreads();
-  fwd_layer(wt + IN * H + H * H, w + OB3, H, act[1], HP, act[2], tid);
+  fwd_layer(wt + IN * HP + H * HP, w + OB3, H, act[1], HP, act[2], tid);
   __syncthreads();
   float l = 0.f;
   if (tid < MLP_SLICE) {     // wave 0: one sample per lane
