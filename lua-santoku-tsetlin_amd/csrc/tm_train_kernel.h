@@ -16,8 +16,11 @@ namespace {
 // Sum over the G consecutive lanes of a clause group, returned to every lane
 // of the group.  Full waves reduce with DPP row shifts and row broadcasts
 // (lane 63 ends with the total); half waves the same, with the broadcast
-// kept inside each half (lanes 31 and 63 end with the totals); smaller
-// groups use butterfly shuffles.
+// kept inside each half (lanes 31 and 63 end with the totals); groups of 16
+// and 8 lanes with a DPP butterfly (the other lane of a pair, the other pair
+// of a quad, the other quad of a half-row by its mirror, and for 16 the
+// other half-row by the row's mirror), which stays in the VALU where a
+// shuffle goes through the LDS crossbar and back.
 template <int G>
 __device__ __forceinline__ uint32_t group_sum(uint32_t v)
 {
@@ -38,6 +41,13 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t v)
     v += __builtin_amdgcn_update_dpp(0u, v, 0x142, 0xA, 0xF, true);   // row_bcast:15 into rows 1, 3
     const uint32_t lo = __builtin_amdgcn_readlane(v, 31), hi = __builtin_amdgcn_readlane(v, 63);
     return (threadIdx.x & 32) ? hi : lo;
+  }
+  if (G == 16 || G == 8) {
+    v += __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xF, 0xF, true);    // quad_perm [1,0,3,2]: lane ^ 1
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xF, 0xF, true);    // quad_perm [2,3,0,1]: lane ^ 2
+    v += __builtin_amdgcn_update_dpp(0u, v, 0x141, 0xF, 0xF, true);   // row_half_mirror: the other quad
+    if (G == 16) v += __builtin_amdgcn_update_dpp(0u, v, 0x140, 0xF, 0xF, true);   // row_mirror: the other 8
+    return v;
   }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
