@@ -59,7 +59,8 @@ def build_sampler(spec, global_dev=None, rng=None):
         shift = (1 - minv) if (is_log and minv <= 0) else 0
         smin, smax = minv + shift, maxv + shift
         span = (math.log(smax) - math.log(smin)) if is_log else (maxv - minv)
-        state = {"jitter": (spec.get("dev") or global_dev or 1.0) * span}
+        j0 = (spec.get("dev") or global_dev or 1.0) * span
+        state = {"jitter": j0, "jitter0": j0}
 
         def sample(center=None):
             if center is not None:

@@ -330,17 +330,23 @@ class Proposer:
     best-scoring candidates (a share `snap` of the coordinates pushed onto the
     nearer bound, where the best settings of a space often lie) are scored and
     added.  The network's picks are the best-scoring candidates not tried
-    before that keep a distance `spread` (unit box) from each other, and a
-    share `explore` of the round stays plain draws."""
+    before: a share `exploit` of them in score order, the rest keeping a
+    distance `spread` (unit box) from each other; a share `explore` of the
+    round stays plain draws.  With `anneal`, the spacing shrinks with the
+    search's own jitter.  (A trial's score is one noisy draw: trials of
+    nearly the same setting differ by several points, and the search keeps
+    the best one.  `exploit` repeats trials near the predicted optimum for
+    that reason.)"""
 
     def __init__(self, ctx, param_names=opt.PARAM_NAMES, seed=0, pool=4096, explore=0.25, min_history=48,
                  steps=300, lr=3e-3, key=opt.make_key, refine=2, refine_top=64, ensemble=4, kappa=0.5, snap=0.15,
-                 spread=0.2, initial=0.5, folds=5):
+                 spread=0.2, initial=0.5, folds=5, anneal=False, exploit=0.0):
         self.ctx, self.names, self.pool, self.explore = dict(ctx), param_names, pool, explore
         self.min_history, self.steps, self.lr, self.key = min_history, steps, lr, key
         self.refine, self.refine_top = refine, refine_top
         self.ensemble, self.kappa, self.snap, self.spread = max(1, ensemble), kappa, snap, spread
         self.initial, self.folds = initial, max(2, folds)
+        self.anneal, self.exploit = anneal, exploit
         self.rng = np.random.default_rng(seed)
         self.seed = seed
         self.last = {"mode": "plain"}
@@ -424,13 +430,26 @@ class Proposer:
             sc = np.concatenate([sc, score(moved)])
         seen = {self.key(p) for p, _ in history}
         n_model = n - int(round(self.explore * n))
+        n_exploit = int(round(self.exploit * n_model))
+        spread = self.spread
+        if self.anneal:
+            # the search's own step size (its jitter, narrowed by the 1/5
+            # success rule) relative to its start scales the picks' spacing
+            for smp in samplers.values():
+                st = getattr(smp, "state", None)
+                if st and st.get("jitter0"):
+                    spread *= st["jitter"] / st["jitter0"]
+                    break
         picks = []
-        U = space.unit(cols) if self.spread > 0 else None
+        U = space.unit(cols) if spread > 0 else None
         chosen = []
         for i in np.argsort(-sc, kind="stable"):
-            # a pick keeps at least `spread` (Euclidean, unit box) from the
-            # round's other picks, so the round covers more than one optimum
-            if U is not None and chosen and np.min(np.sqrt(((U[chosen] - U[i]) ** 2).sum(1))) < self.spread:
+            # past the first `n_exploit` picks, a pick keeps at least `spread`
+            # (Euclidean, unit box) from the round's other spaced picks, so
+            # the round covers more than one optimum
+            spaced = len(picks) >= n_exploit
+            if spaced and U is not None and chosen and \
+                    np.min(np.sqrt(((U[chosen] - U[i]) ** 2).sum(1))) < spread:
                 continue
             p = space.params(cols, i)
             k = self.key(opt._swap_tolerance(dict(p)))
@@ -438,12 +457,13 @@ class Proposer:
                 continue
             seen.add(k)
             picks.append(p)
-            chosen.append(i)
+            if spaced:
+                chosen.append(i)
             if len(picks) >= n_model:
                 break
         n_picked = len(picks)
         picks += plain[: n - len(picks)]
         # the first n_model proposals are the network's picks, the rest plain draws
         self.last = {"mode": "surrogate", "r2": float(r2), "loss": float(loss), "n_model": n_picked,
-                     "pool": int(sc.shape[0])}
+                     "pool": int(sc.shape[0]), "spread": spread}
         return picks
