@@ -407,6 +407,13 @@ int tm_mlp_reset(tm_mlp *m, const float *params);
 /* `steps` Adam steps (beta 0.9/0.999, eps 1e-8) on n samples X[n][12] -> Y[n];
  * *loss receives the mean squared error of the last step (may be NULL). */
 int tm_mlp_train(tm_mlp *m, const float *X, const float *Y, uint32_t n, uint32_t steps, float lr, float *loss);
+/* tm_mlp_train of k distinct networks at once, network i on X[i][n[i]][12] ->
+ * Y[i][n[i]]: up to 16 networks on one device share one launch per pass
+ * (more, or several devices: each network's steps on its own stream), so the
+ * fits run side by side; each ends with the parameters tm_mlp_train would
+ * give it.  loss[i] receives network i's last-step loss (loss may be NULL). */
+int tm_mlp_train_many(tm_mlp *const *nets, uint32_t k, const float *const *X, const float *const *Y,
+                      const uint32_t *n, uint32_t steps, float lr, float *loss);
 /* Gradient of the mean squared error at the current parameters (no update). */
 int tm_mlp_gradient(tm_mlp *m, const float *X, const float *Y, uint32_t n, float *grad, float *loss);
 int tm_mlp_predict(tm_mlp *m, const float *X, uint32_t n, float *Y);

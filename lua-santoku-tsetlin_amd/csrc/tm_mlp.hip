@@ -7,7 +7,9 @@
 //   W1[64][12] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4
 // Training splits the samples into slices of 32, one workgroup each, with the
 // weights (also transposed), activations and deltas of its slice in LDS; the
-// slices' gradient sums meet in a second kernel that applies Adam.
+// slices' gradient sums meet in a second kernel that applies Adam.  Several
+// networks' fits (a search refits an ensemble and its cross-validation folds
+// every round) share those two launches, one grid row per network.
 #include "tsetlin_amd.h"
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -127,7 +129,8 @@ __device__ __forceinline__ void back_layer(const float *w, const float *dn, cons
   }
 }
 
-__global__ __launch_bounds__(MLP_THREADS) void mlp_grad_kernel(mlp_grad_args A)
+// the gradient pass of one slice of one network
+__device__ __forceinline__ void grad_slice(const mlp_grad_args &A, const uint32_t slice)
 {
   __shared__ float w[NP];
   __shared__ float wt[IN * HP + 2 * H * HP];        // W1, W2, W3 transposed, rows of HP floats
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(MLP_THREADS) void mlp_grad_kernel(mlp_grad_args A)
   __shared__ float del[3][MLP_SLICE * HP];          // their deltas
   __shared__ float dout[MLP_SLICE];
   const int tid = threadIdx.x;
-  const uint32_t s0 = blockIdx.x * MLP_SLICE;
+  const uint32_t s0 = slice * MLP_SLICE;
   const uint32_t ns = min((uint32_t) MLP_SLICE, A.n - s0);
   // the parameters as 16-byte loads, all of a thread's in flight together
   // (one at a time, each waiting for the last, they took most of a step)
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(MLP_THREADS) void mlp_grad_kernel(mlp_grad_args A)
   if (tid < 64) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
-    if (tid == 0) A.lpart[blockIdx.x] = l;
+    if (tid == 0) A.lpart[slice] = l;
   }
   __syncthreads();
   for (int idx = tid; idx < MLP_SLICE * H; idx += MLP_THREADS) {
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(MLP_THREADS) void mlp_grad_kernel(mlp_grad_args A)
   back_layer(w + OW2, del[1], act[0], del[0], tid);
   __syncthreads();
   // the slice's gradient sums, one parameter per thread at a time
-  float *part = A.part + (size_t) blockIdx.x * NP;
+  float *part = A.part + (size_t) slice * NP;
   for (int p = tid; p < NP; p += MLP_THREADS) {
     float acc = 0.f;
     if (p < OB1) {
@@ -226,6 +229,22 @@ __global__ __launch_bounds__(MLP_THREADS) void mlp_grad_kernel(mlp_grad_args A)
   }
 }
 
+__global__ __launch_bounds__(MLP_THREADS) void mlp_grad_kernel(mlp_grad_args A) { grad_slice(A, blockIdx.x); }
+
+// Several networks' fits in one launch: network blockIdx.y, slice blockIdx.x
+// (the grid spans the largest network's slices).
+constexpr uint32_t MLP_MANY = 16;
+struct mlp_grad_many_args {
+  mlp_grad_args a[MLP_MANY];
+};
+
+__global__ __launch_bounds__(MLP_THREADS) void mlp_grad_many_kernel(mlp_grad_many_args M)
+{
+  const mlp_grad_args &A = M.a[blockIdx.y];
+  if (blockIdx.x * MLP_SLICE >= A.n) return;
+  grad_slice(A, blockIdx.x);
+}
+
 struct mlp_adam_args {
   float *params, *m, *v;       // weights and Adam moments
   const float *part, *lpart;   // the gradient pass's slice sums
@@ -235,9 +254,8 @@ struct mlp_adam_args {
   float lr, beta1, beta2, eps;
 };
 
-__global__ __launch_bounds__(MLP_THREADS) void mlp_adam_kernel(mlp_adam_args A)
+__device__ __forceinline__ void adam_param(const mlp_adam_args &A, const uint32_t p)
 {
-  const uint32_t p = blockIdx.x * MLP_THREADS + threadIdx.x;
   if (p == 0 && A.loss_out) {
     float l = 0.f;
     for (uint32_t b = 0; b < A.slices; b++) l += A.lpart[b];
@@ -258,6 +276,20 @@ __global__ __launch_bounds__(MLP_THREADS) void mlp_adam_kernel(mlp_adam_args A)
   A.m[p] = mp;
   A.v[p] = vp;
   A.params[p] -= A.lr * (mp / c1) / (sqrtf(vp / c2) + A.eps);
+}
+
+__global__ __launch_bounds__(MLP_THREADS) void mlp_adam_kernel(mlp_adam_args A)
+{
+  adam_param(A, blockIdx.x * MLP_THREADS + threadIdx.x);
+}
+
+struct mlp_adam_many_args {
+  mlp_adam_args a[MLP_MANY];
+};
+
+__global__ __launch_bounds__(MLP_THREADS) void mlp_adam_many_kernel(mlp_adam_many_args M)
+{
+  adam_param(M.a[blockIdx.y], blockIdx.x * MLP_THREADS + threadIdx.x);
 }
 
 } // namespace
@@ -295,31 +327,46 @@ int ensure(tm_mlp *m, size_t n)
   return TM_OK;
 }
 
+// The samples onto the network's buffers and the kernel arguments of its
+// steps; the launches go on the network's own stream.
+int stage(tm_mlp *m, const float *X, const float *Y, uint32_t n, float lr, float *gdev, mlp_grad_args &G,
+          mlp_adam_args &A, hipStream_t stream)
+{
+  MLP_HIP(hipSetDevice(m->device));
+  int rc = ensure(m, n);
+  if (rc) return rc;
+  MLP_HIP(hipMemcpyAsync(m->x, X, (size_t) n * IN * 4, hipMemcpyHostToDevice, stream));
+  MLP_HIP(hipMemcpyAsync(m->y, Y, (size_t) n * 4, hipMemcpyHostToDevice, stream));
+  const uint32_t slices = (n + MLP_SLICE - 1) / MLP_SLICE;
+  G.params = m->params; G.X = m->x; G.Y = m->y; G.part = m->part; G.lpart = m->lpart; G.n = n;
+  A.params = m->params; A.m = m->m; A.v = m->v; A.part = m->part; A.lpart = m->lpart;
+  A.grad_out = gdev; A.loss_out = m->loss; A.slices = slices; A.n = n;
+  A.lr = lr; A.beta1 = 0.9f; A.beta2 = 0.999f; A.eps = 1e-8f;
+  return TM_OK;
+}
+
+void launch_step(tm_mlp *m, const mlp_grad_args &G, mlp_adam_args &A, uint32_t step)
+{
+  A.t = m->step + step + 1;
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(A.slices), dim3(MLP_THREADS), 0, m->stream, G);
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((NP + MLP_THREADS - 1) / MLP_THREADS), dim3(MLP_THREADS), 0, m->stream, A);
+}
+
 int run_train(tm_mlp *m, const float *X, const float *Y, uint32_t n, uint32_t steps, float lr, float *grad,
               float *loss)
 {
   if (!m || !X || !Y || n == 0) return mlp_fail(TM_E_ARG, "mlp: bad arguments");
-  MLP_HIP(hipSetDevice(m->device));
-  int rc = ensure(m, n);
-  if (rc) return rc;
-  MLP_HIP(hipMemcpyAsync(m->x, X, (size_t) n * IN * 4, hipMemcpyHostToDevice, m->stream));
-  MLP_HIP(hipMemcpyAsync(m->y, Y, (size_t) n * 4, hipMemcpyHostToDevice, m->stream));
   float *gdev = nullptr;
-  if (grad) MLP_HIP(hipMalloc(&gdev, NP * sizeof(float)));
-  const uint32_t slices = (n + MLP_SLICE - 1) / MLP_SLICE;
-  mlp_grad_args G;
-  G.params = m->params; G.X = m->x; G.Y = m->y; G.part = m->part; G.lpart = m->lpart; G.n = n;
-  mlp_adam_args A;
-  A.params = m->params; A.m = m->m; A.v = m->v; A.part = m->part; A.lpart = m->lpart;
-  A.grad_out = gdev; A.loss_out = m->loss; A.slices = slices; A.n = n;
-  A.lr = lr; A.beta1 = 0.9f; A.beta2 = 0.999f; A.eps = 1e-8f;
-  const uint32_t nsteps = grad ? 1 : steps;
-  for (uint32_t step = 0; step < nsteps; step++) {
-    A.t = m->step + step + 1;
-    hipLaunchKernelGGL(mlp_grad_kernel, dim3(slices), dim3(MLP_THREADS), 0, m->stream, G);
-    hipLaunchKernelGGL(mlp_adam_kernel, dim3((NP + MLP_THREADS - 1) / MLP_THREADS), dim3(MLP_THREADS), 0, m->stream,
-                       A);
+  if (grad) {
+    MLP_HIP(hipSetDevice(m->device));
+    MLP_HIP(hipMalloc(&gdev, NP * sizeof(float)));
   }
+  mlp_grad_args G;
+  mlp_adam_args A;
+  int rc = stage(m, X, Y, n, lr, gdev, G, A, m->stream);
+  if (rc) { if (gdev) hipFree(gdev); return rc; }
+  const uint32_t nsteps = grad ? 1 : steps;
+  for (uint32_t step = 0; step < nsteps; step++) launch_step(m, G, A, step);
   MLP_HIP(hipGetLastError());
   if (grad) MLP_HIP(hipMemcpyAsync(grad, gdev, NP * sizeof(float), hipMemcpyDeviceToHost, m->stream));
   if (loss) MLP_HIP(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
@@ -373,6 +420,75 @@ int tm_mlp_reset(tm_mlp *m, const float *params)
 int tm_mlp_train(tm_mlp *m, const float *X, const float *Y, uint32_t n, uint32_t steps, float lr, float *loss)
 {
   return run_train(m, X, Y, n, steps, lr, nullptr, loss);
+}
+
+int tm_mlp_train_many(tm_mlp *const *nets, uint32_t k, const float *const *X, const float *const *Y,
+                      const uint32_t *n, uint32_t steps, float lr, float *loss)
+{
+  if (!nets || !X || !Y || !n) return mlp_fail(TM_E_ARG, "mlp train_many: NULL argument");
+  for (uint32_t i = 0; i < k; i++) {
+    if (!nets[i] || !X[i] || !Y[i] || n[i] == 0) return mlp_fail(TM_E_ARG, "mlp train_many: bad arguments for network %u", i);
+    // two fits of one network would share its buffers and its stream
+    for (uint32_t j = 0; j < i; j++)
+      if (nets[j] == nets[i]) return mlp_fail(TM_E_ARG, "mlp train_many: network %u repeats network %u", i, j);
+  }
+  bool one_device = true;
+  for (uint32_t i = 1; i < k; i++) one_device = one_device && nets[i]->device == nets[0]->device;
+  if (k > 0 && k <= MLP_MANY && one_device) {
+    // one launch per pass for all of them, on the first network's stream
+    tm_mlp *m0 = nets[0];
+    const hipStream_t st = m0->stream;
+    mlp_grad_many_args GM = {};
+    mlp_adam_many_args AM = {};
+    uint32_t slices = 0;
+    for (uint32_t i = 0; i < k; i++) {
+      int rc = stage(nets[i], X[i], Y[i], n[i], lr, nullptr, GM.a[i], AM.a[i], st);
+      if (rc) return rc;
+      slices = max(slices, AM.a[i].slices);
+    }
+    MLP_HIP(hipSetDevice(m0->device));
+    for (uint32_t step = 0; step < steps; step++) {
+      for (uint32_t i = 0; i < k; i++) AM.a[i].t = nets[i]->step + step + 1;
+      hipLaunchKernelGGL(mlp_grad_many_kernel, dim3(slices, k), dim3(MLP_THREADS), 0, st, GM);
+      hipLaunchKernelGGL(mlp_adam_many_kernel, dim3((NP + MLP_THREADS - 1) / MLP_THREADS, k), dim3(MLP_THREADS), 0,
+                         st, AM);
+    }
+    MLP_HIP(hipGetLastError());
+    if (loss)
+      for (uint32_t i = 0; i < k; i++)
+        MLP_HIP(hipMemcpyAsync(&loss[i], nets[i]->loss, sizeof(float), hipMemcpyDeviceToHost, st));
+    MLP_HIP(hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < k; i++) nets[i]->step += steps;
+    return TM_OK;
+  }
+  std::vector<mlp_grad_args> G(k);
+  std::vector<mlp_adam_args> A(k);
+  for (uint32_t i = 0; i < k; i++) {
+    int rc = stage(nets[i], X[i], Y[i], n[i], lr, nullptr, G[i], A[i], nets[i]->stream);
+    if (rc) return rc;
+  }
+  // more networks than one launch takes, or on several devices: step by step
+  // across the networks on their own streams, so every stream has work
+  // queued while the host is still launching
+  int cur = -1;
+  for (uint32_t step = 0; step < steps; step++)
+    for (uint32_t i = 0; i < k; i++) {
+      if (nets[i]->device != cur) {
+        cur = nets[i]->device;
+        MLP_HIP(hipSetDevice(cur));
+      }
+      launch_step(nets[i], G[i], A[i], step);
+    }
+  MLP_HIP(hipGetLastError());
+  for (uint32_t i = 0; i < k; i++) {
+    MLP_HIP(hipSetDevice(nets[i]->device));
+    if (loss) MLP_HIP(hipMemcpyAsync(&loss[i], nets[i]->loss, sizeof(float), hipMemcpyDeviceToHost, nets[i]->stream));
+  }
+  for (uint32_t i = 0; i < k; i++) {
+    MLP_HIP(hipStreamSynchronize(nets[i]->stream));
+    nets[i]->step += steps;
+  }
+  return TM_OK;
 }
 
 int tm_mlp_gradient(tm_mlp *m, const float *X, const float *Y, uint32_t n, float *grad, float *loss)

@@ -42,6 +42,8 @@ def _lib():
             "tm_mlp_create": (ctypes.c_int, [fp, ctypes.POINTER(vp)]),
             "tm_mlp_reset": (ctypes.c_int, [vp, fp]),
             "tm_mlp_train": (ctypes.c_int, [vp, fp, fp, u32, u32, ctypes.c_float, fp]),
+            "tm_mlp_train_many": (ctypes.c_int, [ctypes.POINTER(vp), u32, ctypes.POINTER(fp), ctypes.POINTER(fp),
+                                                 ctypes.POINTER(u32), u32, ctypes.c_float, fp]),
             "tm_mlp_gradient": (ctypes.c_int, [vp, fp, fp, u32, fp, fp]),
             "tm_mlp_predict": (ctypes.c_int, [vp, fp, u32, fp]),
             "tm_mlp_params": (ctypes.c_int, [vp, fp]),
@@ -86,6 +88,31 @@ class MLP:
         loss = ctypes.c_float()
         self._check(self._L.tm_mlp_train(self._h, xp, yp, y.shape[0], steps, lr, ctypes.byref(loss)))
         return loss.value
+
+    @staticmethod
+    def train_many(nets, Xs, Ys, steps, lr=3e-3):
+        """net.train(X, Y, steps, lr) for every (net, X, Y), the fits side by
+        side on the networks' own streams; the networks must be distinct.
+        Returns each network's last-step loss."""
+        k = len(nets)
+        if not (len(Xs) == len(Ys) == k):
+            raise ValueError("train_many: %d networks, %d inputs, %d targets" % (k, len(Xs), len(Ys)))
+        if k == 0:
+            return []
+        L = nets[0]._L
+        xs = [_f32(X) for X in Xs]
+        ys = [_f32(Y) for Y in Ys]
+        for (x, _), (y, _) in zip(xs, ys):
+            if x.ndim != 2 or x.shape[1] != INPUTS or y.shape != (x.shape[0],):
+                raise ValueError("train_many: X must be n x %d and Y n" % INPUTS)
+        fp = ctypes.POINTER(ctypes.c_float)
+        hs = (ctypes.c_void_p * k)(*[n._h.value for n in nets])
+        xp = (fp * k)(*[p for _, p in xs])
+        yp = (fp * k)(*[p for _, p in ys])
+        ns = (ctypes.c_uint32 * k)(*[y.shape[0] for y, _ in ys])
+        loss = (ctypes.c_float * k)()
+        nets[0]._check(L.tm_mlp_train_many(hs, k, xp, yp, ns, steps, lr, loss))
+        return list(loss)
 
     def gradient(self, X, Y):
         x, xp = _f32(X)
@@ -134,14 +161,21 @@ def _effective_target(clauses, target, tolerance):
     return out.astype(np.int64)
 
 
+_FEATURE_OPTIONS = (("clauses", 8.0), ("specificity", 1.0), ("target", -1.0), ("clause_tolerance", 1.0),
+                    ("clause_maximum", 1.0), ("include_bits", 1.0))
+
+
+def _option_cols(params_list):
+    """One float array per option over a list of settings (unset -> default)."""
+    return {k: np.array([float(p.get(k, d) if p.get(k, d) is not None else d) for p in params_list])
+            for k, d in _FEATURE_OPTIONS}
+
+
 def features(params, ctx):
     """The 12 surrogate inputs: log clauses, log specificity, target/clauses,
     log target, log tolerance, log maximum, include_bits, epochs, feature
     density, log classes, log features, best dev accuracy so far."""
-    cols = {k: np.array([float(params.get(k, d) if params.get(k, d) is not None else d)])
-            for k, d in (("clauses", 8.0), ("specificity", 1.0), ("target", -1.0), ("clause_tolerance", 1.0),
-                         ("clause_maximum", 1.0), ("include_bits", 1.0))}
-    return [float(v) for v in features_batch(cols, ctx)[0]]
+    return [float(v) for v in features_batch(_option_cols([params]), ctx)[0]]
 
 
 def features_batch(cols, ctx):
@@ -367,32 +401,32 @@ class Proposer:
     def _fit(self, history):
         best = max(s for _, s in history)
         ctx = dict(self.ctx, best=best)
-        X = np.array([features(p, ctx) for p, _ in history], np.float32)
+        X = features_batch(_option_cols([p for p, _ in history]), ctx)
         y = np.array([s for _, s in history], np.float32)
         mu, sd = float(y.mean()), float(y.std() + 1e-6)
         Y = (y - mu) / sd
         # held-out check: out-of-fold predictions of `folds` fits, each
         # against the mean of its own training part
         idx = self.rng.permutation(len(y))
-        resid = total = 0.0
+        folds = []
         for f in range(self.folds):
             test = idx[f::self.folds]
             train = np.setdiff1d(idx, test)
-            if test.size == 0 or train.size == 0:
-                continue
-            net = self._net(0, self.seed + len(y) + f)
-            net.train(X[train], Y[train], self.steps, self.lr)
+            if test.size and train.size:
+                folds.append((f, train, test))
+        sels = [np.arange(len(y)) if e == 0 else self.rng.integers(len(y), size=len(y))
+                for e in range(self.ensemble)]
+        # every fit at once: the fold fits first, then the ensemble members
+        fold_nets = [self._net(i, self.seed + len(y) + f) for i, (f, _, _) in enumerate(folds)]
+        nets = [self._net(len(folds) + e, self.seed + len(y) + 7919 * e) for e in range(self.ensemble)]
+        losses = MLP.train_many(fold_nets + nets, [X[tr] for _, tr, _ in folds] + [X[sel] for sel in sels],
+                                [Y[tr] for _, tr, _ in folds] + [Y[sel] for sel in sels], self.steps, self.lr)
+        resid = total = 0.0
+        for net, (_, train, test) in zip(fold_nets, folds):
             resid += float(((net.predict(X[test]) - Y[test]) ** 2).sum())
             total += float(((Y[test] - Y[train].mean()) ** 2).sum())
         r2 = 1.0 - resid / (total + 1e-9)
-        nets, loss = [], None
-        for e in range(self.ensemble):
-            net = self._net(e, self.seed + len(y) + 7919 * e)
-            sel = np.arange(len(y)) if e == 0 else self.rng.integers(len(y), size=len(y))
-            l = net.train(X[sel], Y[sel], self.steps, self.lr)
-            loss = l if loss is None else loss
-            nets.append(net)
-        return nets, ctx, r2, loss
+        return nets, ctx, r2, losses[len(folds)]
 
     def __call__(self, samplers, n, history):
         plain = [opt.sample_params(samplers, self.names) for _ in range(n)]
@@ -442,14 +476,14 @@ class Proposer:
                     break
         picks = []
         U = space.unit(cols) if spread > 0 else None
-        chosen = []
+        # every candidate's distance to the nearest spaced pick so far
+        near = np.full(sc.shape[0], np.inf) if U is not None else None
         for i in np.argsort(-sc, kind="stable"):
             # past the first `n_exploit` picks, a pick keeps at least `spread`
             # (Euclidean, unit box) from the round's other spaced picks, so
             # the round covers more than one optimum
             spaced = len(picks) >= n_exploit
-            if spaced and U is not None and chosen and \
-                    np.min(np.sqrt(((U[chosen] - U[i]) ** 2).sum(1))) < spread:
+            if spaced and near is not None and near[i] < spread:
                 continue
             p = space.params(cols, i)
             k = self.key(opt._swap_tolerance(dict(p)))
@@ -457,8 +491,8 @@ class Proposer:
                 continue
             seen.add(k)
             picks.append(p)
-            if spaced:
-                chosen.append(i)
+            if spaced and near is not None:
+                np.minimum(near, np.sqrt(((U - U[i]) ** 2).sum(1)), out=near)
             if len(picks) >= n_model:
                 break
         n_picked = len(picks)

@@ -50,6 +50,35 @@ def test_adam_steps_match_oracle_and_reduce_loss():
     assert MO.loss(net.params(), X, Y) < 0.2 * l0
 
 
+def test_train_many_equals_separate_fits():
+    """tm_mlp_train_many: networks fitted side by side on their own streams,
+    each on its own sample count, end bit-equal to the same fits one after
+    another; a network listed twice is refused."""
+    sets = [data(20 + i, n) for i, n in enumerate((300, 77, 448, 32, 1))]
+    inits = [S.init_params(40 + i) for i in range(len(sets))]
+    one = []
+    for p, (X, Y) in zip(inits, sets):
+        net = S.MLP(p)
+        loss = net.train(X, Y, 25, 2e-3)
+        one.append((net.params(), loss))
+        net.destroy()
+    nets = [S.MLP(p) for p in inits]
+    losses = S.MLP.train_many(nets, [X for X, _ in sets], [Y for _, Y in sets], 25, 2e-3)
+    for net, (p1, l1), l in zip(nets, one, losses):
+        assert np.array_equal(net.params(), p1)
+        assert l == l1
+    # a second call continues every network's Adam step count
+    S.MLP.train_many(nets[:2], [sets[0][0], sets[1][0]], [sets[0][1], sets[1][1]], 5, 2e-3)
+    ref = S.MLP(inits[0])
+    ref.train(*sets[0], 25, 2e-3)
+    ref.train(*sets[0], 5, 2e-3)
+    assert np.array_equal(nets[0].params(), ref.params())
+    with pytest.raises(S.TsetlinError):
+        S.MLP.train_many([nets[0], nets[1], nets[0]], [sets[0][0]] * 3, [sets[0][1]] * 3, 1, 1e-3)
+    for net in nets + [ref]:
+        net.destroy()
+
+
 def test_reset_equals_a_fresh_network():
     """tm_mlp_reset: parameters, Adam moments and step count start again, so
     training after a reset equals training a new network (the slice sums
