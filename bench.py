@@ -8,6 +8,7 @@ over the 200k samples (inputs resident in HBM).  `value` is training
 samples/s; predict samples/s over the 20k test rows is reported beside it.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-search] [--no-encoder]
+                    [--search-seed S] [--search-seeds K]
 
 The same line carries workloads 4 and 5 (batched search trials, surrogate-
 guided vs plain search on workload 1's data) under "search_batched_trials"
