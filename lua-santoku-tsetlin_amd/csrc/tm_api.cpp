@@ -1582,10 +1582,16 @@ struct tm_batch {
 namespace {
 
 // Autotuner choices keyed by (row words, state bits, row stride, workgroup
-// count rounded down to a power of two): the candidate tiles depend on the
-// first two, a tile must divide the stride the rows were staged with, and the
-// choice depends on the trial mix mostly through how many workgroups it
-// makes (a search round of a new trial mix then reuses an earlier tuning).
+// count rounded down to a power of two, capped at kTuneBlocksCap): the
+// candidate tiles depend on the first two, a tile must divide the stride the
+// rows were staged with, and the choice depends on the trial mix mostly
+// through how many workgroups it makes, until there are enough to fill the
+// GPU several times over.  A search round of a new trial mix then reuses an
+// earlier tuning: before the cap, the first rounds of a search, whose
+// batches grow from a few thousand chunks to tens of thousands, re-tuned
+// at each new power of two (seven timed epochs each, about a third of a
+// search's wall time).
+constexpr uint64_t kTuneBlocksCap = 4096;
 std::mutex g_tune_mu;
 std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, std::tuple<uint32_t, uint32_t, uint32_t, double>>
     g_tune_cache;
@@ -1739,7 +1745,7 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   uint64_t nblocks = 0;
   for (uint32_t t = 0; t < nt; t++) nblocks += (uint64_t) T[t].clause_chunks * b->m[t]->classes;
   uint32_t nb_bucket = 1;
-  while ((uint64_t) nb_bucket * 2 <= nblocks && nb_bucket < 0x80000000u) nb_bucket *= 2;
+  while ((uint64_t) nb_bucket * 2 <= nblocks && nb_bucket < kTuneBlocksCap) nb_bucket *= 2;
   const auto key = std::make_tuple(m0->wd, m0->state_bits, stride, 0u, nb_bucket);
   bool cached = false;
   {
