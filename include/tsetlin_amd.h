@@ -345,15 +345,17 @@ int tm_encoding_accuracy_dev(const uint8_t *predicted_dev, const uint8_t *expect
 /* ---------------------------------------------------------------------------
  * Batched search trials.  No reference counterpart: optimize.lua runs search
  * trials one after another on one reusable machine (optimize.lua:730-768).
- * Here N classifiers that share data, classes and features but differ in
- * clauses, clause_tolerance, clause_maximum, target, specificity,
+ * Here N classifiers (or N encoders, for the encoder search of
+ * optimize.lua:842) that share data, classes (hidden bits) and features but
+ * differ in clauses, clause_tolerance, clause_maximum, target, specificity,
  * include_bits and seed train together, one launch per epoch for all of
  * them (replayed from a captured HIP graph after the first epoch).  Each
- * trial's automata are bit-identical to a solo tm_train_classifier run with
- * the same options.  The tile shape and workgroup order are chosen by timing
- * the candidates on the first epoch (the autotuner); the choice is cached per
- * (row words, state bits, row stride, workgroup count rounded down to a power
- * of two), so later batches of similar size reuse it.
+ * trial's automata are bit-identical to a solo tm_train_classifier (or
+ * tm_train_encoder) run with the same options.  The tile shape and workgroup
+ * order are chosen by timing the candidates on the first epoch (the
+ * autotuner); the choice is cached per (row words, state bits, row stride,
+ * machine type, workgroup count rounded down to a power of two, capped at
+ * 4096), so later batches of similar size reuse it.
  */
 typedef struct tm_batch tm_batch;
 
@@ -371,6 +373,17 @@ int tm_batch_train_classifier(tm_batch *b, uint32_t n, const uint8_t *problems, 
 int tm_batch_train_classifier_dev(tm_batch *b, uint32_t n, const uint8_t *problems_dev,
                                   const int64_t *solutions_dev, uint32_t iterations, tm_batch_epoch_fn each,
                                   void *user);
+/* A batch of encoder trials (the options' classes are the hidden bits), and
+ * its training on sentences and codes laid out as tm_train_encoder takes
+ * them (codes: [n][(classes + 7) / 8] bytes).  A batch trains only as its own type;
+ * the accuracy calls below score classifier batches (an encoder trial is
+ * scored through tm_batch_machine: tm_predict_encoder_dev and
+ * tm_encoding_accuracy_dev). */
+int tm_batch_create_encoder(const tm_options *opts, uint32_t ntrials, tm_batch **out);
+int tm_batch_train_encoder(tm_batch *b, uint32_t n, const uint8_t *sentences, const uint8_t *codes,
+                           uint32_t iterations, tm_batch_epoch_fn each, void *user);
+int tm_batch_train_encoder_dev(tm_batch *b, uint32_t n, const uint8_t *sentences_dev, const uint8_t *codes_dev,
+                               uint32_t iterations, tm_batch_epoch_fn each, void *user);
 /* Held-out accuracy of every trial on device-resident rows and labels. */
 int tm_batch_accuracy_dev(tm_batch *b, const uint8_t *problems_dev, const int64_t *solutions_dev, uint32_t n,
                           double *accuracy);

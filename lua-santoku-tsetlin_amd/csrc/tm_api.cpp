@@ -1568,11 +1568,12 @@ int tm_predict_classifier_dev_noalloc(tm_machine *m, const uint8_t *problems_dev
 
 struct tm_batch {
   std::vector<tm_machine *> m;
+  int type = TM_CLASSIFIER;      // every trial's machine type
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint32_t wd = 0, input_chunks = 0;
-  dbuf trials, block_map, perm, perm_scratch, ekeys, epoch, stats, rows, words, labels, hits, pred;
+  dbuf trials, block_map, perm, perm_scratch, ekeys, epoch, stats, rows, words, labels, codes, hits, pred;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   uint32_t lanes = 0, wpl = 0, order = 0;
@@ -1623,11 +1624,16 @@ std::vector<uint32_t> batch_map(const tm_batch *b, const std::vector<tm_train_tr
   return map;
 }
 
-int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *labels, uint32_t iterations,
-                tm_batch_epoch_fn each, void *user, bool dev)
+// labels (classifier trials) or codes (encoder trials, [n][class_chunks]
+// bytes as tm_train_encoder takes them): the batch's type picks which.
+int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *labels, const uint8_t *codes,
+                uint32_t iterations, tm_batch_epoch_fn each, void *user, bool dev, int want_type)
 {
   if (!b) return fail(TM_E_ARG, "batch is NULL");
-  if (n > 0 && (!rows || !labels)) return fail(TM_E_ARG, "batch train: problems and solutions must not be NULL");
+  if (b->type != want_type)
+    return fail(TM_E_ARG, "batch train: the batch holds %s trials", b->type == TM_CLASSIFIER ? "classifier" : "encoder");
+  if (n > 0 && (!rows || (want_type == TM_CLASSIFIER ? !labels : !codes)))
+    return fail(TM_E_ARG, "batch train: problems and %s must not be NULL", want_type == TM_CLASSIFIER ? "solutions" : "codes");
   HIP_TRY(hipSetDevice(b->device));
   const uint32_t nt = (uint32_t) b->m.size();
   tm_machine *m0 = b->m[0];
@@ -1651,11 +1657,22 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   const uint32_t *words;
   int rc;
   if ((rc = stage_rows(m0, rows, n, dev, stride, b->rows, b->words, &words))) return rc;
-  const int64_t *dlabels = labels;
-  if (!dev) {
-    if ((rc = b->labels.ensure((size_t) n * 8))) return rc;
-    HIP_TRY(hipMemcpyAsync(b->labels.p, labels, (size_t) n * 8, hipMemcpyHostToDevice, b->stream));
-    dlabels = b->labels.as<int64_t>();
+  const int64_t *dlabels = nullptr;
+  const uint8_t *dcodes = nullptr;
+  if (want_type == TM_CLASSIFIER) {
+    dlabels = labels;
+    if (!dev) {
+      if ((rc = b->labels.ensure((size_t) n * 8))) return rc;
+      HIP_TRY(hipMemcpyAsync(b->labels.p, labels, (size_t) n * 8, hipMemcpyHostToDevice, b->stream));
+      dlabels = b->labels.as<int64_t>();
+    }
+  } else {
+    dcodes = codes;
+    if (!dev) {
+      if ((rc = b->codes.ensure((size_t) n * m0->class_chunks))) return rc;
+      HIP_TRY(hipMemcpyAsync(b->codes.p, codes, (size_t) n * m0->class_chunks, hipMemcpyHostToDevice, b->stream));
+      dcodes = b->codes.as<uint8_t>();
+    }
   }
   // per-trial sample orders and epoch keys for every epoch
   const size_t per = (size_t) iterations * n;
@@ -1715,6 +1732,8 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   L.epoch_dev = b->epoch.as<uint32_t>();
   L.rows = words;
   L.labels = dlabels;
+  L.codes = dcodes;
+  L.encoder = want_type == TM_ENCODER;
   L.n = n;
   L.wd = m0->wd;
   L.row_stride = stride;
@@ -1746,7 +1765,7 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   for (uint32_t t = 0; t < nt; t++) nblocks += (uint64_t) T[t].clause_chunks * b->m[t]->classes;
   uint32_t nb_bucket = 1;
   while ((uint64_t) nb_bucket * 2 <= nblocks && nb_bucket < kTuneBlocksCap) nb_bucket *= 2;
-  const auto key = std::make_tuple(m0->wd, m0->state_bits, stride, 0u, nb_bucket);
+  const auto key = std::make_tuple(m0->wd, m0->state_bits, stride, (uint32_t) b->type, nb_bucket);
   bool cached = false;
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
@@ -1840,11 +1859,7 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   return TM_OK;
 }
 
-} // namespace
-
-extern "C" {
-
-int tm_batch_create(const tm_options *opts, uint32_t ntrials, tm_batch **out)
+int batch_create(const tm_options *opts, uint32_t ntrials, int type, tm_batch **out)
 {
   if (!out) return fail(TM_E_ARG, "batch create: out is NULL");
   *out = nullptr;
@@ -1867,14 +1882,29 @@ int tm_batch_create(const tm_options *opts, uint32_t ntrials, tm_batch **out)
     tm_options o = opts[t];
     o.reusable = 1;
     tm_machine *m = nullptr;
-    rc = create_common(&o, TM_CLASSIFIER, &m, b->stream);
+    rc = create_common(&o, type, &m, b->stream);
     if (rc == TM_OK) b->m.push_back(m);
   }
   if (rc) { tm_batch_destroy(b); return rc; }
+  b->type = type;
   b->wd = b->m[0]->wd;
   b->input_chunks = b->m[0]->input_chunks;
   *out = b;
   return TM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int tm_batch_create(const tm_options *opts, uint32_t ntrials, tm_batch **out)
+{
+  return batch_create(opts, ntrials, TM_CLASSIFIER, out);
+}
+
+int tm_batch_create_encoder(const tm_options *opts, uint32_t ntrials, tm_batch **out)
+{
+  return batch_create(opts, ntrials, TM_ENCODER, out);
 }
 
 int tm_batch_machine(tm_batch *b, uint32_t trial, tm_machine **out)
@@ -1888,19 +1918,32 @@ int tm_batch_machine(tm_batch *b, uint32_t trial, tm_machine **out)
 int tm_batch_train_classifier(tm_batch *b, uint32_t n, const uint8_t *problems, const int64_t *solutions,
                               uint32_t iterations, tm_batch_epoch_fn each, void *user)
 {
-  return batch_train(b, n, problems, solutions, iterations, each, user, false);
+  return batch_train(b, n, problems, solutions, nullptr, iterations, each, user, false, TM_CLASSIFIER);
 }
 
 int tm_batch_train_classifier_dev(tm_batch *b, uint32_t n, const uint8_t *problems, const int64_t *solutions,
                                   uint32_t iterations, tm_batch_epoch_fn each, void *user)
 {
-  return batch_train(b, n, problems, solutions, iterations, each, user, true);
+  return batch_train(b, n, problems, solutions, nullptr, iterations, each, user, true, TM_CLASSIFIER);
+}
+
+int tm_batch_train_encoder(tm_batch *b, uint32_t n, const uint8_t *sentences, const uint8_t *codes,
+                           uint32_t iterations, tm_batch_epoch_fn each, void *user)
+{
+  return batch_train(b, n, sentences, nullptr, codes, iterations, each, user, false, TM_ENCODER);
+}
+
+int tm_batch_train_encoder_dev(tm_batch *b, uint32_t n, const uint8_t *sentences_dev, const uint8_t *codes_dev,
+                               uint32_t iterations, tm_batch_epoch_fn each, void *user)
+{
+  return batch_train(b, n, sentences_dev, nullptr, codes_dev, iterations, each, user, true, TM_ENCODER);
 }
 
 int tm_batch_accuracy_dev(tm_batch *b, const uint8_t *problems, const int64_t *solutions, uint32_t n,
                           double *accuracy)
 {
   if (!b || !accuracy) return fail(TM_E_ARG, "batch accuracy: NULL argument");
+  if (b->type != TM_CLASSIFIER) return fail(TM_E_ARG, "batch accuracy: the batch holds encoder trials");
   if (n == 0) { for (size_t t = 0; t < b->m.size(); t++) accuracy[t] = 0; return TM_OK; }
   if (!problems || !solutions) return fail(TM_E_ARG, "batch accuracy: problems and solutions must not be NULL");
   HIP_TRY(hipSetDevice(b->device));
@@ -1929,6 +1972,7 @@ int tm_batch_class_accuracy_dev(tm_batch *b, const uint8_t *problems, const int6
                                 double *macro)
 {
   if (!b || !macro) return fail(TM_E_ARG, "batch class accuracy: NULL argument");
+  if (b->type != TM_CLASSIFIER) return fail(TM_E_ARG, "batch class accuracy: the batch holds encoder trials");
   const uint32_t nt = (uint32_t) b->m.size(), K = b->m[0]->classes;
   if (n == 0) {
     for (size_t i = 0; i < (size_t) 3 * nt; i++) macro[i] = 0;

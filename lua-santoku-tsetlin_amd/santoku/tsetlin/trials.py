@@ -18,15 +18,19 @@ from . import capi as _c
 
 
 class Batch:
-    def __init__(self, base, trials):
+    def __init__(self, base, trials, encoder=False):
         """base: options shared by every trial (classes, features, negative, ...);
-        trials: list of per-trial option dicts merged over base."""
+        trials: list of per-trial option dicts merged over base.  encoder:
+        the trials are encoders (classes = hidden bits), trained on sentences
+        and codes (optimize.lua:842's search)."""
         self._lib = _c.load_library()
+        self.encoder = bool(encoder)
         opts = (_c.Options * len(trials))()
         for i, t in enumerate(trials):
             opts[i] = _options(dict(base, **t))
         h = ctypes.c_void_p()
-        _check(self._lib.tm_batch_create(opts, len(trials), ctypes.byref(h)))
+        create = self._lib.tm_batch_create_encoder if self.encoder else self._lib.tm_batch_create
+        _check(create(opts, len(trials), ctypes.byref(h)))
         self._h = h
         self.n = len(trials)
         self._machines = []
@@ -43,7 +47,9 @@ class Batch:
         return self._machines[t]
 
     def train(self, samples, problems, solutions, iterations, each=None):
-        """each(epoch, active) -> False stops all; clearing active[t] freezes trial t."""
+        """each(epoch, active) -> False stops all; clearing active[t] freezes
+        trial t.  For an encoder batch, problems are the sentences and
+        solutions the packed codes ([n][(classes + 7) // 8] bytes)."""
         keep = []
 
         def on_epoch(_u, epoch, active_ptr):
@@ -56,7 +62,18 @@ class Batch:
             return 0 if r is False else 1
 
         cb = _c.BATCH_EPOCH_FN(on_epoch) if each is not None else _c.BATCH_EPOCH_FN()
-        if _is_device_tensor(problems):
+        if self.encoder:
+            if _is_device_tensor(problems):
+                info = self._machines[0].info()
+                pp = _device_buffer(problems, np.uint8, int(samples) * info["input_chunks"], "batch train: sentences")
+                cp = _device_buffer(solutions, np.uint8, int(samples) * info["class_chunks"], "batch train: codes")
+                _check(self._lib.tm_batch_train_encoder_dev(self._h, int(samples), pp, cp, int(iterations), cb, None))
+            else:
+                ra, rp = _host_buffer(problems)
+                ca, cp = _host_buffer(solutions)
+                keep += [ra, ca]
+                _check(self._lib.tm_batch_train_encoder(self._h, int(samples), rp, cp, int(iterations), cb, None))
+        elif _is_device_tensor(problems):
             row = self._machines[0].info()["input_chunks"]
             pp = _device_buffer(problems, np.uint8, int(samples) * row, "batch train: problems")
             lp = _device_buffer(solutions, np.int64, int(samples) * 8, "batch train: solutions")

@@ -189,3 +189,57 @@ def test_batch_trial_fn_restores_best_epoch():
         # the driver swaps an inverted tolerance / maximum (optimize.lua:731-733)
         ref = solo(dict(opt._swap_tolerance(dict(params[t])), seed=300 + t), d["train_rows"], d["train_y"], e)
         assert np.array_equal(b.machine(t).checkpoint(), ref.checkpoint()), (t, e)
+
+
+# ---------------------------------------------------------------------------
+# encoder trials (the encoder search, optimize.lua:842): a batch of encoders
+# on one sentence/code set, every trial equal to its solo tm_train_encoder run
+
+ENC_BASE = dict(classes=16, features=128)
+
+
+def test_encoder_batch_bitexact_vs_solo():
+    import torch
+    d = D.projection_codes_task(21, n=600, visible=128, hidden=16)
+    rng = np.random.default_rng(8)
+    trials = trial_opts(rng, 5)
+    trials[1]["specificity"] = 1.5          # many draws per event: repeat hits on the levels
+    b = TR.Batch(ENC_BASE, trials, encoder=True)
+    b.train(600, d["rows"], d["codes"], 3)
+    for t, o in enumerate(trials):
+        m = tm.encoder(dict(ENC_BASE, reusable=True, **o))
+        m.train(samples=600, sentences=d["rows"], codes=d["codes"], iterations=3)
+        assert_same_machine(b.machine(t), m, "encoder trial %d" % t)
+        assert np.array_equal(b.machine(t).predict(d["rows"][:64], 64), m.predict(d["rows"][:64], 64))
+    # device-resident sentences and codes, and an early stop through the callback
+    rows = torch.from_numpy(d["rows"]).cuda()
+    codes = torch.from_numpy(d["codes"]).cuda()
+    b2 = TR.Batch(ENC_BASE, trials, encoder=True)
+
+    def each(epoch, active):
+        if epoch == 1:
+            active[3] = 0
+        return True
+
+    b2.train(600, rows, codes, 3, each)
+    for t, o in enumerate(trials):
+        m = tm.encoder(dict(ENC_BASE, reusable=True, **o))
+        m.train(samples=600, sentences=d["rows"], codes=d["codes"], iterations=1 if t == 3 else 3)
+        assert_same_machine(b2.machine(t), m, "encoder trial %d (device inputs)" % t)
+
+
+def test_batch_type_is_enforced():
+    import torch
+    from santoku.tsetlin import TsetlinError
+    d = D.projection_codes_task(22, n=64, visible=128, hidden=16)
+    b = TR.Batch(ENC_BASE, trial_opts(np.random.default_rng(9), 2), encoder=True)
+    rows = torch.from_numpy(d["rows"]).cuda()
+    labels = torch.zeros(64, dtype=torch.int64).cuda()
+    with pytest.raises(TsetlinError):
+        b.class_accuracy(rows, labels, 64)
+    c = TR.Batch(BASE, trial_opts(np.random.default_rng(9), 2))
+    c.encoder = True                        # a classifier batch called as an encoder one
+    crow = torch.zeros((64, c.machine(0).info()["input_chunks"]), dtype=torch.uint8).cuda()
+    ccodes = torch.zeros((64, c.machine(0).info()["class_chunks"]), dtype=torch.uint8).cuda()
+    with pytest.raises(TsetlinError, match="classifier trials"):
+        c.train(64, crow, ccodes, 1)
