@@ -9,6 +9,7 @@ samples/s; predict samples/s over the 20k test rows is reported beside it.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-search] [--no-encoder]
                     [--search-seed S] [--search-seeds K]
+    python bench.py --encoder-trials      # batched encoder search trials against one after another
 
 The same line carries workloads 4 and 5 (batched search trials, surrogate-
 guided vs plain search on workload 1's data) under "search_batched_trials"
@@ -328,6 +329,64 @@ SEARCH_SPACE = {
 }
 
 
+def encoder_trials_measure(ntrials=32, epochs=5, seed=3):
+    """Encoder search trials (optimize.lua:842's search) batched against one
+    after another: 8192 samples of 1024 visible bits (density 0.1) coded into
+    32 hidden bits, `ntrials` encoders with clauses log-uniform 64..1024 per
+    hidden bit, specificity 2..40, target clauses/8 or /4, tolerance and
+    maximum log-uniform 1..64; every trial compared with its solo machine."""
+    import numpy as np
+    import torch
+    import santoku.tsetlin as tm
+    from santoku.tsetlin import data as D
+    from santoku.tsetlin import optimize as opt
+    from santoku.tsetlin import trials as TR
+    space = dict(SEARCH_SPACE, clauses={"def": 256, "min": 64, "max": 1024, "log": True, "round": 8})
+    d = D.projection_codes_task(seed, n=8192, visible=1024, hidden=32)
+    dev = torch.device("cuda:0")
+    rows = torch.from_numpy(d["rows"]).to(dev)
+    codes = torch.from_numpy(d["codes"]).to(dev)
+    n = rows.shape[0]
+    base = dict(classes=32, features=1024)
+    samplers = opt.build_samplers(space, opt.PARAM_NAMES, 1.0, np.random.default_rng(seed))
+
+    def draw():
+        p = {k: (s.sample(None) if s.type == "range" else s.sample()) for k, s in samplers.items() if s is not None}
+        return opt._swap_tolerance(p)
+
+    trials = [dict(draw(), seed=7000 + i) for i in range(ntrials)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solo = []
+    for t in trials:
+        m = tm.encoder(dict(base, reusable=True, **t))      # keeps its counters for the comparison
+        m.train(samples=n, sentences=rows, codes=codes, iterations=epochs)
+        solo.append(m)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    runs = []
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b = TR.Batch(base, trials, encoder=True)
+        b.train(n, rows, codes, epochs)
+        torch.cuda.synchronize()
+        runs.append(time.perf_counter() - t0)
+        log("encoder trials: batched run %d: %.2f s, tuning %s" % (rep, runs[-1], b.tuning()))
+    exact = sum(int(np.array_equal(b.machine(i).checkpoint(), solo[i].checkpoint())
+                    and np.array_equal(b.machine(i).counts(), solo[i].counts())) for i in range(ntrials))
+    log("encoder trials: %d x %d epochs one after another %.2f s, batched %.2f s" % (ntrials, epochs, t_seq, runs[-1]))
+    return {
+        "workload": "%d encoder trials on 8192 samples, 1024 visible -> 32 hidden bits, %d epochs: clauses "
+                    "log-uniform 64..1024 per bit, specificity 2..40, target clauses/8 or /4, tolerance and "
+                    "maximum log-uniform 1..64" % (ntrials, epochs),
+        "batched_s": round(runs[-1], 3), "batched_first_run_s": round(runs[0], 3), "sequential_s": round(t_seq, 3),
+        "batched_speedup": round(t_seq / runs[-1], 2),
+        "trials_bitexact_vs_solo": "%d/%d" % (exact, ntrials),
+        "autotuner": b.tuning(),
+    }
+
+
 def search_measure(seed=1, search_seed=9, search_seeds=8):
     """Workloads 4 and 5 on workload 1's data (8192 train / 2048 dev rows, 256
     features, 4 classes, 10 epochs per trial).
@@ -487,6 +546,8 @@ def main():
     ap.add_argument("--search-seed", type=int, default=9, help="first seed of workload 5")
     ap.add_argument("--encoder", action="store_true", help="workload 3 only: the 256-bit encoder at full machine shape")
     ap.add_argument("--no-encoder", action="store_true", help="leave workload 3 out of the default line")
+    ap.add_argument("--encoder-trials", action="store_true",
+                    help="batched against one after another encoder search trials (prints its own line)")
     ap.add_argument("--enc-train", type=int, default=16_384)
     ap.add_argument("--enc-predict", type=int, default=500_000)
     ap.add_argument("--data", help=argparse.SUPPRESS)
@@ -518,6 +579,9 @@ def main():
         sys.exit(2)
     if args.encoder:
         print(json.dumps(encoder_bench(args)), flush=True)
+        return
+    if args.encoder_trials:
+        print(json.dumps(encoder_trials_measure()), flush=True)
         return
     if args.copy_calibration:
         import torch
