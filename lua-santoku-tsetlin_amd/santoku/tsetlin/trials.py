@@ -127,7 +127,7 @@ class Batch:
 
 
 def batch_trial_fn(base, train_rows, train_y, dev_rows, dev_y, iterations, patience=None, tolerance=1e-8,
-                   seed=0, metric="f1"):
+                   seed=0, metric="f1", encoder=False):
     """A batch_fn for optimize.search_batched: every trial of a round trains
     in one Batch, scored after each epoch with the reference's search metric
     (macro F1 of evaluator.class_accuracy on the dev rows, as the classifier
@@ -137,19 +137,30 @@ def batch_trial_fn(base, train_rows, train_y, dev_rows, dev_y, iterations, patie
     `patience` epochs without improvement freeze it, and after training every
     checkpointed trial is restored to its best epoch and scored again, which is
     its result.  Without patience there is no checkpoint and a trial's result
-    is its last epoch's score.  Trial t takes seed + t."""
+    is its last epoch's score.  Trial t takes seed + t.
+
+    encoder=True: the trials are encoders (base's classes = hidden bits) on
+    sentences train_rows and packed codes train_y, scored by the encoder
+    specs' search metric, evaluator.encoding_accuracy's mean_hamming (one
+    minus the mean bit error) of each trial's codes for dev_rows against
+    dev_y (encoder/mnist.lua:368-372)."""
     import torch
 
+    from . import evaluator
     from .optimize import _swap_tolerance
 
     def score(b):
+        if encoder:
+            n = dev_rows.shape[0]
+            return np.array([evaluator.encoding_accuracy(b.machine(t).predict(dev_rows, n), dev_y, n,
+                                                         base["classes"])["mean_hamming"] for t in range(b.n)])
         if metric == "accuracy":
             return b.accuracy(dev_rows, dev_y, dev_y.shape[0])
         return b.class_accuracy(dev_rows, dev_y, dev_y.shape[0])[:, 2]
 
     def fn(batch_params, info):
         trials = [dict(_swap_tolerance(dict(p)), seed=seed + i) for i, p in enumerate(batch_params)]
-        b = Batch(base, trials)
+        b = Batch(base, trials, encoder=encoder)
         nt = len(trials)
         best = np.full(nt, -np.inf)
         last = np.full(nt, -np.inf)

@@ -243,3 +243,31 @@ def test_batch_type_is_enforced():
     ccodes = torch.zeros((64, c.machine(0).info()["class_chunks"]), dtype=torch.uint8).cuda()
     with pytest.raises(TsetlinError, match="classifier trials"):
         c.train(64, crow, ccodes, 1)
+
+
+def test_encoder_trial_fn_scores_and_restores_best_epoch():
+    """The encoder search's batch function: per-epoch mean_hamming of each
+    trial's codes on the dev rows (the encoder specs' search_metric,
+    encoder/mnist.lua:368-372), the best epoch restored; each trial's result
+    equals the evaluator on a solo encoder trained that many epochs."""
+    import torch
+    from santoku.tsetlin import evaluator as EV
+    from santoku.tsetlin import optimize as opt
+    d = D.projection_codes_task(23, n=900, visible=128, hidden=16)
+    rows = torch.from_numpy(d["rows"][:700]).cuda()
+    codes = torch.from_numpy(d["codes"][:700]).cuda()
+    drows = torch.from_numpy(d["rows"][700:]).cuda()
+    dcodes = torch.from_numpy(d["codes"][700:]).cuda()
+    trials = trial_opts(np.random.default_rng(16), 5)
+    params = [{k: v for k, v in t.items() if k != "seed"} for t in trials]
+    fn = TR.batch_trial_fn(ENC_BASE, rows, codes, drows, dcodes, iterations=6, patience=2, seed=500, encoder=True)
+    scores = fn(params, {"round": 1})
+    assert np.array_equal(np.array(scores), fn.last_best)
+    for t in range(5):
+        e = int(fn.last_best_epoch[t])
+        assert 1 <= e <= 6
+        m = tm.encoder(dict(ENC_BASE, reusable=True, **dict(opt._swap_tolerance(dict(params[t])), seed=500 + t)))
+        m.train(samples=700, sentences=d["rows"][:700], codes=d["codes"][:700], iterations=e)
+        assert np.array_equal(fn.last_batch.machine(t).checkpoint(), m.checkpoint()), (t, e)
+        want = EV.encoding_accuracy(m.predict(d["rows"][700:], 200), d["codes"][700:], 200, 16)["mean_hamming"]
+        assert scores[t] == want, (t, scores[t], want)
