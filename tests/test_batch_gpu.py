@@ -226,6 +226,15 @@ def test_encoder_batch_bitexact_vs_solo():
         m = tm.encoder(dict(ENC_BASE, reusable=True, **o))
         m.train(samples=600, sentences=d["rows"], codes=d["codes"], iterations=1 if t == 3 else 3)
         assert_same_machine(b2.machine(t), m, "encoder trial %d (device inputs)" % t)
+    # six state bits and include_bits 2 (another train object, another include test)
+    base6 = dict(ENC_BASE, state=6)
+    trials6 = [dict(o, include_bits=2) for o in trials[:3]]
+    b3 = TR.Batch(base6, trials6, encoder=True)
+    b3.train(600, d["rows"], d["codes"], 2)
+    for t, o in enumerate(trials6):
+        m = tm.encoder(dict(base6, reusable=True, **o))
+        m.train(samples=600, sentences=d["rows"], codes=d["codes"], iterations=2)
+        assert_same_machine(b3.machine(t), m, "encoder trial %d (6 state bits)" % t)
 
 
 def test_batch_type_is_enforced():
