@@ -77,6 +77,16 @@ def test_train_many_equals_separate_fits():
         S.MLP.train_many([nets[0], nets[1], nets[0]], [sets[0][0]] * 3, [sets[0][1]] * 3, 1, 1e-3)
     for net in nets + [ref]:
         net.destroy()
+    # more networks than one launch takes (16): each on its own stream
+    many = [S.MLP(S.init_params(60 + i)) for i in range(17)]
+    X, Y = data(30, 90)
+    S.MLP.train_many(many, [X] * 17, [Y] * 17, 6, 2e-3)
+    for i, net in enumerate(many):
+        ref = S.MLP(S.init_params(60 + i))
+        ref.train(X, Y, 6, 2e-3)
+        assert np.array_equal(net.params(), ref.params()), i
+        ref.destroy()
+        net.destroy()
 
 
 def test_reset_equals_a_fresh_network():
