@@ -384,6 +384,30 @@ int tm_batch_train_encoder(tm_batch *b, uint32_t n, const uint8_t *sentences, co
                            uint32_t iterations, tm_batch_epoch_fn each, void *user);
 int tm_batch_train_encoder_dev(tm_batch *b, uint32_t n, const uint8_t *sentences_dev, const uint8_t *codes_dev,
                                uint32_t iterations, tm_batch_epoch_fn each, void *user);
+/* Batches of individualized trials (the default of the reference's encoder
+ * and newsgroups specs, encoder/mnist.lua:23, classify/newsgroups.lua:29):
+ * every trial takes the same feat_offsets (classes + 1 offsets, as
+ * tm_create_classifier_ind / tm_create_encoder_ind), and training takes the
+ * problems' byte size and dim_offsets as tm_train_classifier_ind /
+ * tm_train_encoder_ind do.  One launch per epoch trains every class of every
+ * trial; each trial equals its solo individualized machine.  These batches
+ * are scored through tm_batch_machine (the *_ind predict calls). */
+int tm_batch_create_classifier_ind(const tm_options *opts, uint32_t ntrials, const int64_t *feat_offsets,
+                                   uint32_t n_offsets, tm_batch **out);
+int tm_batch_create_encoder_ind(const tm_options *opts, uint32_t ntrials, const int64_t *feat_offsets,
+                                uint32_t n_offsets, tm_batch **out);
+int tm_batch_train_classifier_ind(tm_batch *b, uint32_t n, const uint8_t *problems, size_t problems_bytes,
+                                  const int64_t *solutions, const int64_t *dim_offsets, uint32_t iterations,
+                                  tm_batch_epoch_fn each, void *user);
+int tm_batch_train_classifier_ind_dev(tm_batch *b, uint32_t n, const uint8_t *problems_dev, size_t problems_bytes,
+                                      const int64_t *solutions_dev, const int64_t *dim_offsets, uint32_t iterations,
+                                      tm_batch_epoch_fn each, void *user);
+int tm_batch_train_encoder_ind(tm_batch *b, uint32_t n, const uint8_t *sentences, size_t sentences_bytes,
+                               const uint8_t *codes, const int64_t *dim_offsets, uint32_t iterations,
+                               tm_batch_epoch_fn each, void *user);
+int tm_batch_train_encoder_ind_dev(tm_batch *b, uint32_t n, const uint8_t *sentences_dev, size_t sentences_bytes,
+                                   const uint8_t *codes_dev, const int64_t *dim_offsets, uint32_t iterations,
+                                   tm_batch_epoch_fn each, void *user);
 /* Held-out accuracy of every trial on device-resident rows and labels. */
 int tm_batch_accuracy_dev(tm_batch *b, const uint8_t *problems_dev, const int64_t *solutions_dev, uint32_t n,
                           double *accuracy);

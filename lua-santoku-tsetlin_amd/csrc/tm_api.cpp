@@ -1569,6 +1569,7 @@ int tm_predict_classifier_dev_noalloc(tm_machine *m, const uint8_t *problems_dev
 struct tm_batch {
   std::vector<tm_machine *> m;
   int type = TM_CLASSIFIER;      // every trial's machine type
+  bool ind = false;              // individualized trials (one feature set per class, shared by the trials)
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1607,7 +1608,11 @@ void batch_release_graph(tm_batch *b)
 
 // Workgroup map (trial, chunk): order 0 is trial-major; order 1 starts with
 // the trials of most random-literal draws per event (the heaviest chunks).
-std::vector<uint32_t> batch_map(const tm_batch *b, const std::vector<tm_train_trial> &T, uint32_t order)
+// (table entry, chunk) per workgroup; entry e has chunks[e] chunks (a plain
+// trial: all of its classes' chunks; an individualized trial's class: that
+// class's chunks)
+std::vector<uint32_t> batch_map(const std::vector<tm_train_trial> &T, const std::vector<uint32_t> &chunks,
+                                uint32_t order)
 {
   std::vector<uint32_t> idx(T.size());
   for (uint32_t t = 0; t < idx.size(); t++) idx[t] = t;
@@ -1618,18 +1623,23 @@ std::vector<uint32_t> batch_map(const tm_batch *b, const std::vector<tm_train_tr
   std::vector<uint32_t> map;
   for (uint32_t t : idx) {
     if (!T[t].active) continue;
-    const uint32_t chunks = T[t].clause_chunks * b->m[t]->classes;
-    for (uint32_t c = 0; c < chunks; c++) { map.push_back(t); map.push_back(c); }
+    for (uint32_t c = 0; c < chunks[t]; c++) { map.push_back(t); map.push_back(c); }
   }
   return map;
 }
 
 // labels (classifier trials) or codes (encoder trials, [n][class_chunks]
 // bytes as tm_train_encoder takes them): the batch's type picks which.
-int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *labels, const uint8_t *codes,
-                uint32_t iterations, tm_batch_epoch_fn each, void *user, bool dev, int want_type)
+// Individualized batches take the problems' byte size and dim_offsets (class
+// h's rows start at byte dim_offsets[h], as tm_train_classifier_ind).
+int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, size_t rows_bytes, const int64_t *dim_offsets,
+                const int64_t *labels, const uint8_t *codes, uint32_t iterations, tm_batch_epoch_fn each, void *user,
+                bool dev, int want_type)
 {
   if (!b) return fail(TM_E_ARG, "batch is NULL");
+  if (b->ind != (dim_offsets != nullptr))
+    return fail(TM_E_ARG, b->ind ? "batch train: an individualized batch takes dim_offsets"
+                                 : "batch train: dim_offsets given for a batch that is not individualized");
   if (b->type != want_type)
     return fail(TM_E_ARG, "batch train: the batch holds %s trials", b->type == TM_CLASSIFIER ? "classifier" : "encoder");
   if (n > 0 && (!rows || (want_type == TM_CLASSIFIER ? !labels : !codes)))
@@ -1640,9 +1650,11 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   b->train_ms = 0;
   for (tm_machine *m : b->m) { m->predict_ready = false; m->trained = false; memset(m->stats, 0, sizeof m->stats); }
   if (iterations == 0 || n == 0) {
+    // setup fills every word alike: the whole state in one call (also the
+    // ragged per-class layout of an individualized machine)
     for (tm_machine *m : b->m)
-      HIP_TRY(tm_launch_setup(m->counts.as<uint32_t>(), m->actions.as<uint32_t>(), m->nclauses(), m->state_bits - 1,
-                              m->wd, b->stream));
+      HIP_TRY(tm_launch_setup(m->counts.as<uint32_t>(), m->actions.as<uint32_t>(), m->action_words(),
+                              m->state_bits - 1, 1, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     for (tm_machine *m : b->m) m->trained = true;
     return TM_OK;
@@ -1656,7 +1668,21 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   const uint32_t stride = (m0->wd + maxw - 1) / maxw * maxw;
   const uint32_t *words;
   int rc;
-  if ((rc = stage_rows(m0, rows, n, dev, stride, b->rows, b->words, &words))) return rc;
+  std::vector<uint32_t> cstrides;     // individualized: per-class row strides and word offsets
+  std::vector<uint64_t> woff;
+  if (b->ind) {
+    // every class's rows once for all trials, each stride a multiple of
+    // every candidate tile's words per lane
+    size_t extent;
+    if ((rc = ind_extent(m0, n, dim_offsets, rows_bytes, "batch train", &extent))) return rc;
+    cstrides.resize(m0->classes);
+    for (uint32_t h = 0; h < m0->classes; h++) cstrides[h] = (m0->ind[h].wd + maxw - 1) / maxw * maxw;
+    if ((rc = stage_rows_ind(m0, rows, n, extent, dim_offsets, cstrides, b->rows, b->words, woff, dev))) return rc;
+    HIP_TRY(hipStreamSynchronize(m0->stream));
+    words = b->words.as<uint32_t>();
+  } else if ((rc = stage_rows(m0, rows, n, dev, stride, b->rows, b->words, &words))) {
+    return rc;
+  }
   const int64_t *dlabels = nullptr;
   const uint8_t *dcodes = nullptr;
   if (want_type == TM_CLASSIFIER) {
@@ -1722,6 +1748,41 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
     P.class_base = 0;
     P.chunk_base = 0;
   }
+  // The launch's table: one entry per plain trial, or per (trial, class) of
+  // individualized trials, each class entry as a solo individualized machine
+  // trains it (its state offsets, rows, width, masks and random-literal
+  // count; its chunks draw as chunks h * clause_chunks + c of the stream).
+  std::vector<tm_train_trial> E;
+  std::vector<uint32_t> etrial, echunks;
+  for (uint32_t t = 0; t < nt; t++) {
+    tm_machine *m = b->m[t];
+    if (!b->ind) {
+      E.push_back(T[t]);
+      etrial.push_back(t);
+      echunks.push_back(T[t].clause_chunks * m->classes);
+      continue;
+    }
+    for (uint32_t h = 0; h < m->classes; h++) {
+      const tm_ind_class &c = m->ind[h];
+      tm_train_trial Q = T[t];
+      Q.counts = m->counts.as<uint32_t>() + c.coff;
+      Q.actions = m->actions.as<uint32_t>() + c.aoff;
+      Q.s_draws = (uint32_t)((double)(2 * c.features) / m->specificity);
+      Q.rows = words + woff[h];
+      Q.row_stride = cstrides[h];
+      Q.wd = c.wd;
+      Q.input_bits = c.input_bits;
+      Q.last_live = live_mask(c.wd, c.input_chunks, c.tail_mask);
+      Q.last_eval = eval_mask(c.wd, c.input_bits);
+      Q.class_base = h;
+      Q.chunk_base = h * m->clause_chunks;
+      E.push_back(Q);
+      etrial.push_back(t);
+      echunks.push_back(m->clause_chunks);
+    }
+  }
+  const size_t ne = E.size();
+  if ((rc = b->trials.ensure(ne * sizeof(tm_train_trial)))) return rc;
   HIP_TRY(hipMemcpyAsync(b->ekeys.p, ek.data(), ek.size() * 4, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(tm_launch_zero(b->stats.p, (size_t) nt * TM_NSTATS * 8, b->stream));
   HIP_TRY(tm_launch_zero(b->epoch.p, 256, b->stream));
@@ -1745,13 +1806,13 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   L.last_eval = m0->last_eval_mask();
   L.all_ib1 = 1;
   L.all_full = 1;
-  for (auto &P : T) {
+  for (auto &P : E) {
     L.all_ib1 &= P.include_bits <= 1;
     L.all_full &= full_rows(P, L.wd);
   }
 
   auto upload_map = [&](uint32_t order) -> int {
-    std::vector<uint32_t> map = batch_map(b, T, order);
+    std::vector<uint32_t> map = batch_map(E, echunks, order);
     int r = b->block_map.ensure(map.size() * 4 + 8);
     if (r) return r;
     HIP_TRY(hipMemcpyAsync(b->block_map.p, map.data(), map.size() * 4, hipMemcpyHostToDevice, b->stream));
@@ -1762,10 +1823,10 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
 
   // autotune on epoch 0 (every candidate computes the same epoch-0 state)
   uint64_t nblocks = 0;
-  for (uint32_t t = 0; t < nt; t++) nblocks += (uint64_t) T[t].clause_chunks * b->m[t]->classes;
+  for (uint32_t c : echunks) nblocks += c;
   uint32_t nb_bucket = 1;
   while ((uint64_t) nb_bucket * 2 <= nblocks && nb_bucket < kTuneBlocksCap) nb_bucket *= 2;
-  const auto key = std::make_tuple(m0->wd, m0->state_bits, stride, (uint32_t) b->type, nb_bucket);
+  const auto key = std::make_tuple(m0->wd, m0->state_bits, stride, (uint32_t) b->type + (b->ind ? 2u : 0u), nb_bucket);
   bool cached = false;
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
@@ -1783,7 +1844,7 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
     }
   }
   if (!cached) {
-    HIP_TRY(hipMemcpyAsync(b->trials.p, T.data(), nt * sizeof(tm_train_trial), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->trials.p, E.data(), ne * sizeof(tm_train_trial), hipMemcpyHostToDevice, b->stream));
     double best = 1e300;
     auto time_epoch = [&](uint32_t lanes, uint32_t wpl, uint32_t order) -> int {
       L.lanes = lanes;
@@ -1809,8 +1870,8 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
     std::lock_guard<std::mutex> g(g_tune_mu);
     g_tune_cache[key] = std::make_tuple(b->lanes, b->wpl, b->order, b->tuned_ms);
   }
-  for (uint32_t t = 0; t < nt; t++) T[t].stats = b->stats.as<unsigned long long>() + (size_t) t * TM_NSTATS;
-  HIP_TRY(hipMemcpyAsync(b->trials.p, T.data(), nt * sizeof(tm_train_trial), hipMemcpyHostToDevice, b->stream));
+  for (size_t e = 0; e < ne; e++) E[e].stats = b->stats.as<unsigned long long>() + (size_t) etrial[e] * TM_NSTATS;
+  HIP_TRY(hipMemcpyAsync(b->trials.p, E.data(), ne * sizeof(tm_train_trial), hipMemcpyHostToDevice, b->stream));
   if ((rc = upload_map(b->order))) return rc;
   L.lanes = b->lanes;
   L.wpl = b->wpl;
@@ -1842,9 +1903,13 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
     bool any = false;
     for (uint32_t t = 0; t < nt; t++) {
       if (active[t] != prev[t]) {
+        // every table entry of trial t (its classes, when individualized)
         uint32_t a = active[t] ? 1u : 0u;
-        HIP_TRY(hipMemcpyAsync((char *) b->trials.p + t * sizeof(tm_train_trial) + offsetof(tm_train_trial, active),
-                               &a, 4, hipMemcpyHostToDevice, b->stream));
+        for (size_t e = 0; e < ne; e++)
+          if (etrial[e] == t)
+            HIP_TRY(hipMemcpyAsync((char *) b->trials.p + e * sizeof(tm_train_trial) +
+                                       offsetof(tm_train_trial, active),
+                                   &a, 4, hipMemcpyHostToDevice, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
         prev[t] = active[t];
       }
@@ -1859,7 +1924,10 @@ int batch_train(tm_batch *b, uint32_t n, const uint8_t *rows, const int64_t *lab
   return TM_OK;
 }
 
-int batch_create(const tm_options *opts, uint32_t ntrials, int type, tm_batch **out)
+// feat_offsets (individualized trials): classes + 1 offsets, one feature set
+// per class shared by every trial, as tm_create_classifier_ind takes them.
+int batch_create(const tm_options *opts, uint32_t ntrials, int type, const int64_t *feat_offsets, uint32_t n_offsets,
+                 tm_batch **out)
 {
   if (!out) return fail(TM_E_ARG, "batch create: out is NULL");
   *out = nullptr;
@@ -1882,11 +1950,12 @@ int batch_create(const tm_options *opts, uint32_t ntrials, int type, tm_batch **
     tm_options o = opts[t];
     o.reusable = 1;
     tm_machine *m = nullptr;
-    rc = create_common(&o, type, &m, b->stream);
+    rc = feat_offsets ? create_ind(&o, feat_offsets, n_offsets, type, &m) : create_common(&o, type, &m, b->stream);
     if (rc == TM_OK) b->m.push_back(m);
   }
   if (rc) { tm_batch_destroy(b); return rc; }
   b->type = type;
+  b->ind = feat_offsets != nullptr;
   b->wd = b->m[0]->wd;
   b->input_chunks = b->m[0]->input_chunks;
   *out = b;
@@ -1899,12 +1968,26 @@ extern "C" {
 
 int tm_batch_create(const tm_options *opts, uint32_t ntrials, tm_batch **out)
 {
-  return batch_create(opts, ntrials, TM_CLASSIFIER, out);
+  return batch_create(opts, ntrials, TM_CLASSIFIER, nullptr, 0, out);
 }
 
 int tm_batch_create_encoder(const tm_options *opts, uint32_t ntrials, tm_batch **out)
 {
-  return batch_create(opts, ntrials, TM_ENCODER, out);
+  return batch_create(opts, ntrials, TM_ENCODER, nullptr, 0, out);
+}
+
+int tm_batch_create_classifier_ind(const tm_options *opts, uint32_t ntrials, const int64_t *feat_offsets,
+                                   uint32_t n_offsets, tm_batch **out)
+{
+  if (!feat_offsets) return fail(TM_E_ARG, "batch create: feat_offsets must not be NULL");
+  return batch_create(opts, ntrials, TM_CLASSIFIER, feat_offsets, n_offsets, out);
+}
+
+int tm_batch_create_encoder_ind(const tm_options *opts, uint32_t ntrials, const int64_t *feat_offsets,
+                                uint32_t n_offsets, tm_batch **out)
+{
+  if (!feat_offsets) return fail(TM_E_ARG, "batch create: feat_offsets must not be NULL");
+  return batch_create(opts, ntrials, TM_ENCODER, feat_offsets, n_offsets, out);
 }
 
 int tm_batch_machine(tm_batch *b, uint32_t trial, tm_machine **out)
@@ -1918,25 +2001,61 @@ int tm_batch_machine(tm_batch *b, uint32_t trial, tm_machine **out)
 int tm_batch_train_classifier(tm_batch *b, uint32_t n, const uint8_t *problems, const int64_t *solutions,
                               uint32_t iterations, tm_batch_epoch_fn each, void *user)
 {
-  return batch_train(b, n, problems, solutions, nullptr, iterations, each, user, false, TM_CLASSIFIER);
+  return batch_train(b, n, problems, 0, nullptr, solutions, nullptr, iterations, each, user, false, TM_CLASSIFIER);
 }
 
 int tm_batch_train_classifier_dev(tm_batch *b, uint32_t n, const uint8_t *problems, const int64_t *solutions,
                                   uint32_t iterations, tm_batch_epoch_fn each, void *user)
 {
-  return batch_train(b, n, problems, solutions, nullptr, iterations, each, user, true, TM_CLASSIFIER);
+  return batch_train(b, n, problems, 0, nullptr, solutions, nullptr, iterations, each, user, true, TM_CLASSIFIER);
 }
 
 int tm_batch_train_encoder(tm_batch *b, uint32_t n, const uint8_t *sentences, const uint8_t *codes,
                            uint32_t iterations, tm_batch_epoch_fn each, void *user)
 {
-  return batch_train(b, n, sentences, nullptr, codes, iterations, each, user, false, TM_ENCODER);
+  return batch_train(b, n, sentences, 0, nullptr, nullptr, codes, iterations, each, user, false, TM_ENCODER);
 }
 
 int tm_batch_train_encoder_dev(tm_batch *b, uint32_t n, const uint8_t *sentences_dev, const uint8_t *codes_dev,
                                uint32_t iterations, tm_batch_epoch_fn each, void *user)
 {
-  return batch_train(b, n, sentences_dev, nullptr, codes_dev, iterations, each, user, true, TM_ENCODER);
+  return batch_train(b, n, sentences_dev, 0, nullptr, nullptr, codes_dev, iterations, each, user, true, TM_ENCODER);
+}
+
+int tm_batch_train_classifier_ind(tm_batch *b, uint32_t n, const uint8_t *problems, size_t problems_bytes,
+                                  const int64_t *solutions, const int64_t *dim_offsets, uint32_t iterations,
+                                  tm_batch_epoch_fn each, void *user)
+{
+  if (!dim_offsets) return fail(TM_E_ARG, "batch train: dim_offsets must not be NULL");
+  return batch_train(b, n, problems, problems_bytes, dim_offsets, solutions, nullptr, iterations, each, user, false,
+                     TM_CLASSIFIER);
+}
+
+int tm_batch_train_classifier_ind_dev(tm_batch *b, uint32_t n, const uint8_t *problems_dev, size_t problems_bytes,
+                                      const int64_t *solutions_dev, const int64_t *dim_offsets, uint32_t iterations,
+                                      tm_batch_epoch_fn each, void *user)
+{
+  if (!dim_offsets) return fail(TM_E_ARG, "batch train: dim_offsets must not be NULL");
+  return batch_train(b, n, problems_dev, problems_bytes, dim_offsets, solutions_dev, nullptr, iterations, each, user,
+                     true, TM_CLASSIFIER);
+}
+
+int tm_batch_train_encoder_ind(tm_batch *b, uint32_t n, const uint8_t *sentences, size_t sentences_bytes,
+                               const uint8_t *codes, const int64_t *dim_offsets, uint32_t iterations,
+                               tm_batch_epoch_fn each, void *user)
+{
+  if (!dim_offsets) return fail(TM_E_ARG, "batch train: dim_offsets must not be NULL");
+  return batch_train(b, n, sentences, sentences_bytes, dim_offsets, nullptr, codes, iterations, each, user, false,
+                     TM_ENCODER);
+}
+
+int tm_batch_train_encoder_ind_dev(tm_batch *b, uint32_t n, const uint8_t *sentences_dev, size_t sentences_bytes,
+                                   const uint8_t *codes_dev, const int64_t *dim_offsets, uint32_t iterations,
+                                   tm_batch_epoch_fn each, void *user)
+{
+  if (!dim_offsets) return fail(TM_E_ARG, "batch train: dim_offsets must not be NULL");
+  return batch_train(b, n, sentences_dev, sentences_bytes, dim_offsets, nullptr, codes_dev, iterations, each, user,
+                     true, TM_ENCODER);
 }
 
 int tm_batch_accuracy_dev(tm_batch *b, const uint8_t *problems, const int64_t *solutions, uint32_t n,
@@ -1944,6 +2063,7 @@ int tm_batch_accuracy_dev(tm_batch *b, const uint8_t *problems, const int64_t *s
 {
   if (!b || !accuracy) return fail(TM_E_ARG, "batch accuracy: NULL argument");
   if (b->type != TM_CLASSIFIER) return fail(TM_E_ARG, "batch accuracy: the batch holds encoder trials");
+  if (b->ind) return fail(TM_E_ARG, "batch accuracy: individualized trials are scored through tm_batch_machine");
   if (n == 0) { for (size_t t = 0; t < b->m.size(); t++) accuracy[t] = 0; return TM_OK; }
   if (!problems || !solutions) return fail(TM_E_ARG, "batch accuracy: problems and solutions must not be NULL");
   HIP_TRY(hipSetDevice(b->device));
@@ -1973,6 +2093,7 @@ int tm_batch_class_accuracy_dev(tm_batch *b, const uint8_t *problems, const int6
 {
   if (!b || !macro) return fail(TM_E_ARG, "batch class accuracy: NULL argument");
   if (b->type != TM_CLASSIFIER) return fail(TM_E_ARG, "batch class accuracy: the batch holds encoder trials");
+  if (b->ind) return fail(TM_E_ARG, "batch class accuracy: individualized trials are scored through tm_batch_machine");
   const uint32_t nt = (uint32_t) b->m.size(), K = b->m[0]->classes;
   if (n == 0) {
     for (size_t i = 0; i < (size_t) 3 * nt; i++) macro[i] = 0;

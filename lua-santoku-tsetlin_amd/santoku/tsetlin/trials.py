@@ -18,19 +18,28 @@ from . import capi as _c
 
 
 class Batch:
-    def __init__(self, base, trials, encoder=False):
+    def __init__(self, base, trials, encoder=False, feat_offsets=None):
         """base: options shared by every trial (classes, features, negative, ...);
         trials: list of per-trial option dicts merged over base.  encoder:
         the trials are encoders (classes = hidden bits), trained on sentences
-        and codes (optimize.lua:842's search)."""
+        and codes (optimize.lua:842's search).  feat_offsets: individualized
+        trials, one feature set per class shared by all of them (classes + 1
+        offsets, as create(..., individualized=True) takes them); train then
+        takes dim_offsets."""
         self._lib = _c.load_library()
         self.encoder = bool(encoder)
+        self.individualized = feat_offsets is not None
         opts = (_c.Options * len(trials))()
         for i, t in enumerate(trials):
             opts[i] = _options(dict(base, **t))
         h = ctypes.c_void_p()
-        create = self._lib.tm_batch_create_encoder if self.encoder else self._lib.tm_batch_create
-        _check(create(opts, len(trials), ctypes.byref(h)))
+        if self.individualized:
+            fa, fp = _host_buffer(feat_offsets, np.int64)
+            create = self._lib.tm_batch_create_encoder_ind if self.encoder else self._lib.tm_batch_create_classifier_ind
+            _check(create(opts, len(trials), fp, fa.size, ctypes.byref(h)))
+        else:
+            create = self._lib.tm_batch_create_encoder if self.encoder else self._lib.tm_batch_create
+            _check(create(opts, len(trials), ctypes.byref(h)))
         self._h = h
         self.n = len(trials)
         self._machines = []
@@ -46,10 +55,12 @@ class Batch:
         """Borrowed machine of trial t (predict / checkpoint / persist)."""
         return self._machines[t]
 
-    def train(self, samples, problems, solutions, iterations, each=None):
+    def train(self, samples, problems, solutions, iterations, each=None, dim_offsets=None):
         """each(epoch, active) -> False stops all; clearing active[t] freezes
         trial t.  For an encoder batch, problems are the sentences and
-        solutions the packed codes ([n][(classes + 7) // 8] bytes)."""
+        solutions the packed codes ([n][(classes + 7) // 8] bytes).  An
+        individualized batch takes dim_offsets (class h's rows start at byte
+        dim_offsets[h] of problems, as Machine.train)."""
         keep = []
 
         def on_epoch(_u, epoch, active_ptr):
@@ -62,7 +73,31 @@ class Batch:
             return 0 if r is False else 1
 
         cb = _c.BATCH_EPOCH_FN(on_epoch) if each is not None else _c.BATCH_EPOCH_FN()
-        if self.encoder:
+        if self.individualized:
+            if dim_offsets is None:
+                raise TsetlinError("batch train: dim_offsets is required for individualized trials")
+            da, dp = _host_buffer(dim_offsets, np.int64)
+            keep += [da]
+            n = int(samples)
+            if _is_device_tensor(problems):
+                # the library checks dim_offsets and n against the byte size
+                problems = problems.contiguous()
+                pp = _device_buffer(problems, np.uint8, 0, "batch train: problems")
+                if self.encoder:
+                    tp = _device_buffer(solutions, np.uint8, n * self._machines[0].info()["class_chunks"],
+                                        "batch train: codes")
+                    fn = self._lib.tm_batch_train_encoder_ind_dev
+                else:
+                    tp = _device_buffer(solutions, np.int64, n * 8, "batch train: solutions")
+                    fn = self._lib.tm_batch_train_classifier_ind_dev
+                _check(fn(self._h, n, pp, problems.numel(), tp, dp, int(iterations), cb, None))
+            else:
+                ra, rp = _host_buffer(problems)
+                ta, tp = _host_buffer(solutions, np.uint8 if self.encoder else np.int64)
+                keep += [ra, ta]
+                fn = self._lib.tm_batch_train_encoder_ind if self.encoder else self._lib.tm_batch_train_classifier_ind
+                _check(fn(self._h, n, rp, ra.nbytes, tp, dp, int(iterations), cb, None))
+        elif self.encoder:
             if _is_device_tensor(problems):
                 info = self._machines[0].info()
                 pp = _device_buffer(problems, np.uint8, int(samples) * info["input_chunks"], "batch train: sentences")

@@ -280,3 +280,76 @@ def test_encoder_trial_fn_scores_and_restores_best_epoch():
         assert np.array_equal(fn.last_batch.machine(t).checkpoint(), m.checkpoint()), (t, e)
         want = EV.encoding_accuracy(m.predict(d["rows"][700:], 200), d["codes"][700:], 200, 16)["mean_hamming"]
         assert scores[t] == want, (t, scores[t], want)
+
+
+# ---------------------------------------------------------------------------
+# individualized trials (the default of the reference's encoder and
+# newsgroups specs): every class of every trial in one launch per epoch, each
+# trial equal to its solo individualized machine
+
+def _ragged(widths, n, rng):
+    parts, offs, o = [], [], 0
+    for k in widths:
+        b = D.bits_to_cvec(rng.random((n, k)) < 0.3).reshape(-1)
+        parts.append(b)
+        offs.append(o)
+        o += b.size
+    return np.concatenate(parts), np.array(offs, np.int64)
+
+
+@pytest.mark.parametrize("kind,widths", [("classifier", [5, 40, 64, 200]), ("encoder", [63, 9, 300, 17, 130])])
+def test_individualized_batch_bitexact_vs_solo(kind, widths):
+    import torch
+    rng = np.random.default_rng(sum(widths))
+    n, K = 240, len(widths)
+    prob, offs = _ragged(widths, n, rng)
+    feat = np.concatenate([[0], np.cumsum(widths)]).astype(np.int64)
+    enc = kind == "encoder"
+    tgt = (rng.integers(0, 256, (n, (K + 7) // 8), dtype=np.uint8) if enc
+           else rng.integers(0, K, n).astype(np.int64))
+    trials = trial_opts(np.random.default_rng(17), 4)
+    trials[2]["specificity"] = 1.5
+    base = dict(classes=K, features=1)
+    b = TR.Batch(base, trials, encoder=enc, feat_offsets=feat)
+    b.train(n, prob, tgt, 3, dim_offsets=offs)
+
+    def solo_ind(o, epochs):
+        m = tm.create(kind, dict(base, reusable=True, individualized=True, feat_offsets=feat, **o))
+        if enc:
+            m.train(samples=n, sentences=prob, codes=tgt, dim_offsets=offs, iterations=epochs)
+        else:
+            m.train(samples=n, problems=prob, solutions=tgt, dim_offsets=offs, iterations=epochs)
+        return m
+
+    for t, o in enumerate(trials):
+        m = solo_ind(o, 3)
+        assert_same_machine(b.machine(t), m, "%s ind trial %d" % (kind, t))
+        assert np.array_equal(b.machine(t).predict(prob, offs, n), m.predict(prob, offs, n))
+    # device inputs and a trial frozen after its first epoch
+    b2 = TR.Batch(base, trials, encoder=enc, feat_offsets=feat)
+
+    def each(epoch, active):
+        if epoch == 1:
+            active[1] = 0
+        return True
+
+    b2.train(n, torch.from_numpy(prob).cuda(), torch.from_numpy(tgt).cuda(), 3, each, dim_offsets=offs)
+    for t, o in enumerate(trials):
+        assert_same_machine(b2.machine(t), solo_ind(o, 1 if t == 1 else 3), "%s ind trial %d (device)" % (kind, t))
+
+
+def test_individualized_batch_arguments():
+    from santoku.tsetlin import TsetlinError
+    rng = np.random.default_rng(3)
+    prob, offs = _ragged([10, 20], 32, rng)
+    feat = np.array([0, 10, 30], np.int64)
+    b = TR.Batch(dict(classes=2, features=1), trial_opts(rng, 2), feat_offsets=feat)
+    labels = rng.integers(0, 2, 32).astype(np.int64)
+    with pytest.raises(TsetlinError):
+        b.train(32, prob, labels, 1)                                    # no dim_offsets
+    with pytest.raises(TsetlinError, match="bytes"):
+        b.train(32, prob[:-1], labels, 1, dim_offsets=offs)             # problems too short for n rows
+    c = TR.Batch(dict(classes=2, features=16), trial_opts(rng, 2))
+    c.individualized = True                                            # a plain batch given dim_offsets
+    with pytest.raises(TsetlinError, match="not individualized"):
+        c.train(32, np.zeros((32, 4), np.uint8), labels, 1, dim_offsets=offs)
