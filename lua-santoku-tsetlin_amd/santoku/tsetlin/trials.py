@@ -162,7 +162,7 @@ class Batch:
 
 
 def batch_trial_fn(base, train_rows, train_y, dev_rows, dev_y, iterations, patience=None, tolerance=1e-8,
-                   seed=0, metric="f1", encoder=False):
+                   seed=0, metric="f1", encoder=False, feat_offsets=None, dim_offsets=None, dev_dim_offsets=None):
     """A batch_fn for optimize.search_batched: every trial of a round trains
     in one Batch, scored after each epoch with the reference's search metric
     (macro F1 of evaluator.class_accuracy on the dev rows, as the classifier
@@ -178,24 +178,46 @@ def batch_trial_fn(base, train_rows, train_y, dev_rows, dev_y, iterations, patie
     sentences train_rows and packed codes train_y, scored by the encoder
     specs' search metric, evaluator.encoding_accuracy's mean_hamming (one
     minus the mean bit error) of each trial's codes for dev_rows against
-    dev_y (encoder/mnist.lua:368-372)."""
+    dev_y (encoder/mnist.lua:368-372).
+
+    feat_offsets: individualized trials sharing those per-class feature
+    sets; train_rows and dev_rows are then individualized problems buffers
+    read at dim_offsets and dev_dim_offsets, and each trial is scored through
+    its own machine's individualized predict."""
     import torch
 
     from . import evaluator
     from .optimize import _swap_tolerance
 
+    ind = feat_offsets is not None
+    if ind and (dim_offsets is None or dev_dim_offsets is None):
+        raise TsetlinError("batch_trial_fn: individualized trials need dim_offsets and dev_dim_offsets")
+    n_dev = dev_y.shape[0]
+
+    def predict(m):
+        return m.predict(dev_rows, dev_dim_offsets, n_dev) if ind else m.predict(dev_rows, n_dev)
+
     def score(b):
         if encoder:
-            n = dev_rows.shape[0]
-            return np.array([evaluator.encoding_accuracy(b.machine(t).predict(dev_rows, n), dev_y, n,
+            return np.array([evaluator.encoding_accuracy(predict(b.machine(t)), dev_y, n_dev,
                                                          base["classes"])["mean_hamming"] for t in range(b.n)])
+        if ind:
+            col = 2 if metric != "accuracy" else None
+            out = []
+            for t in range(b.n):
+                pred = predict(b.machine(t))
+                if col is None:
+                    out.append(float((pred == dev_y).sum()) / n_dev)
+                else:
+                    out.append(evaluator.class_accuracy(pred, dev_y, n_dev, base["classes"])["f1"])
+            return np.array(out)
         if metric == "accuracy":
             return b.accuracy(dev_rows, dev_y, dev_y.shape[0])
         return b.class_accuracy(dev_rows, dev_y, dev_y.shape[0])[:, 2]
 
     def fn(batch_params, info):
         trials = [dict(_swap_tolerance(dict(p)), seed=seed + i) for i, p in enumerate(batch_params)]
-        b = Batch(base, trials, encoder=encoder)
+        b = Batch(base, trials, encoder=encoder, feat_offsets=feat_offsets)
         nt = len(trials)
         best = np.full(nt, -np.inf)
         last = np.full(nt, -np.inf)
@@ -223,7 +245,7 @@ def batch_trial_fn(base, train_rows, train_y, dev_rows, dev_y, iterations, patie
                     active[t] = 0
             return True
 
-        b.train(train_rows.shape[0], train_rows, train_y, iterations, each)
+        b.train(train_y.shape[0], train_rows, train_y, iterations, each, dim_offsets=dim_offsets)
         if any(c is not None for c in ckpt):
             for t in range(nt):
                 if ckpt[t] is not None:

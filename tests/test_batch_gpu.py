@@ -353,3 +353,37 @@ def test_individualized_batch_arguments():
     c.individualized = True                                            # a plain batch given dim_offsets
     with pytest.raises(TsetlinError, match="not individualized"):
         c.train(32, np.zeros((32, 4), np.uint8), labels, 1, dim_offsets=offs)
+
+
+def test_individualized_trial_fn_restores_best_epoch():
+    """The search's batch function over individualized classifier trials:
+    per-epoch macro F1 through each trial's individualized predict on the dev
+    buffer, the best epoch restored; each result equals the evaluator on a
+    solo individualized machine trained to that epoch."""
+    import torch
+    from santoku.tsetlin import evaluator as EV
+    from santoku.tsetlin import optimize as opt
+    rng = np.random.default_rng(41)
+    widths, n, nd = [12, 70, 33], 400, 150
+    K = len(widths)
+    prob, offs = _ragged(widths, n, rng)
+    dprob, doffs = _ragged(widths, nd, rng)
+    y = rng.integers(0, K, n).astype(np.int64)
+    dy = rng.integers(0, K, nd).astype(np.int64)
+    feat = np.concatenate([[0], np.cumsum(widths)]).astype(np.int64)
+    base = dict(classes=K, features=1)
+    params = [{k: v for k, v in t.items() if k != "seed"} for t in trial_opts(np.random.default_rng(42), 4)]
+    fn = TR.batch_trial_fn(base, torch.from_numpy(prob).cuda(), torch.from_numpy(y).cuda(),
+                           torch.from_numpy(dprob).cuda(), torch.from_numpy(dy).cuda(), iterations=6, patience=2,
+                           seed=900, feat_offsets=feat, dim_offsets=offs, dev_dim_offsets=doffs)
+    scores = fn(params, {"round": 1})
+    assert np.array_equal(np.array(scores), fn.last_best)
+    for t in range(4):
+        e = int(fn.last_best_epoch[t])
+        assert 1 <= e <= 6
+        o = dict(opt._swap_tolerance(dict(params[t])), seed=900 + t)
+        m = tm.create("classifier", dict(base, reusable=True, individualized=True, feat_offsets=feat, **o))
+        m.train(samples=n, problems=prob, solutions=y, dim_offsets=offs, iterations=e)
+        assert np.array_equal(fn.last_batch.machine(t).checkpoint(), m.checkpoint()), (t, e)
+        want = EV.class_accuracy(m.predict(dprob, doffs, nd), dy, nd, K)["f1"]
+        assert scores[t] == want, (t, scores[t], want)
