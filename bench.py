@@ -10,6 +10,7 @@ samples/s; predict samples/s over the 20k test rows is reported beside it.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-search] [--no-encoder]
                     [--search-seed S] [--search-seeds K]
     python bench.py --encoder-trials      # batched encoder search trials against one after another
+    python bench.py --ind-trials          # batched individualized classifier trials against one after another
 
 The same line carries workloads 4 and 5 (batched search trials, surrogate-
 guided vs plain search on workload 1's data) under "search_batched_trials"
@@ -387,6 +388,75 @@ def encoder_trials_measure(ntrials=32, epochs=5, seed=3):
     }
 
 
+def ind_trials_measure(ntrials=32, epochs=10, seed=4):
+    """Individualized classifier search trials (the mode the reference's
+    newsgroups spec searches in, classify/newsgroups.lua:29) batched against
+    one after another: 8192 samples, 20 classes, class h reading its own
+    feature set of 64..512 features (density 0.1, labels drawn uniformly),
+    `ntrials` trials over the workload-4 ranges with clauses log-uniform
+    64..1024 per class; every trial compared with its solo machine."""
+    import numpy as np
+    import torch
+    import santoku.tsetlin as tm
+    from santoku.tsetlin import data as D
+    from santoku.tsetlin import optimize as opt
+    from santoku.tsetlin import trials as TR
+    rng = np.random.default_rng(seed)
+    n, K = 8192, 20
+    widths = rng.integers(64, 513, K)
+    parts, offs, o = [], [], 0
+    for k in widths:
+        b = D.bits_to_cvec(rng.random((n, int(k))) < 0.1).reshape(-1)
+        parts.append(b)
+        offs.append(o)
+        o += b.size
+    dev = torch.device("cuda:0")
+    prob = torch.from_numpy(np.concatenate(parts)).to(dev)
+    offs = np.array(offs, np.int64)
+    y = torch.from_numpy(rng.integers(0, K, n).astype(np.int64)).to(dev)
+    feat = np.concatenate([[0], np.cumsum(widths)]).astype(np.int64)
+    space = dict(SEARCH_SPACE, clauses={"def": 256, "min": 64, "max": 1024, "log": True, "round": 8})
+    samplers = opt.build_samplers(space, opt.PARAM_NAMES, 1.0, np.random.default_rng(seed))
+
+    def draw():
+        p = {k: (s.sample(None) if s.type == "range" else s.sample()) for k, s in samplers.items() if s is not None}
+        return opt._swap_tolerance(p)
+
+    trials = [dict(draw(), seed=8000 + i) for i in range(ntrials)]
+    base = dict(classes=K, features=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solo = []
+    for t in trials:
+        m = tm.create("classifier", dict(base, reusable=True, individualized=True, feat_offsets=feat, **t))
+        m.train(samples=n, problems=prob, solutions=y, dim_offsets=offs, iterations=epochs)
+        solo.append(m)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    runs = []
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b = TR.Batch(base, trials, feat_offsets=feat)
+        b.train(n, prob, y, epochs, dim_offsets=offs)
+        torch.cuda.synchronize()
+        runs.append(time.perf_counter() - t0)
+        log("individualized trials: batched run %d: %.2f s, tuning %s" % (rep, runs[-1], b.tuning()))
+    exact = sum(int(np.array_equal(b.machine(i).checkpoint(), solo[i].checkpoint())
+                    and np.array_equal(b.machine(i).counts(), solo[i].counts())) for i in range(ntrials))
+    log("individualized trials: %d x %d epochs one after another %.2f s, batched %.2f s" % (
+        ntrials, epochs, t_seq, runs[-1]))
+    return {
+        "workload": "%d individualized classifier trials on 8192 samples, 20 classes of 64..512 features each, "
+                    "%d epochs: clauses log-uniform 64..1024 per class, specificity 2..40, target clauses/8 or /4, "
+                    "tolerance and maximum log-uniform 1..64" % (ntrials, epochs),
+        "batched_s": round(runs[-1], 3), "batched_first_run_s": round(runs[0], 3), "sequential_s": round(t_seq, 3),
+        "batched_speedup": round(t_seq / runs[-1], 2),
+        "trials_bitexact_vs_solo": "%d/%d" % (exact, ntrials),
+        "autotuner": b.tuning(),
+    }
+
+
 def search_measure(seed=1, search_seed=9, search_seeds=8):
     """Workloads 4 and 5 on workload 1's data (8192 train / 2048 dev rows, 256
     features, 4 classes, 10 epochs per trial).
@@ -548,6 +618,8 @@ def main():
     ap.add_argument("--no-encoder", action="store_true", help="leave workload 3 out of the default line")
     ap.add_argument("--encoder-trials", action="store_true",
                     help="batched against one after another encoder search trials (prints its own line)")
+    ap.add_argument("--ind-trials", action="store_true",
+                    help="batched against one after another individualized search trials (prints its own line)")
     ap.add_argument("--enc-train", type=int, default=16_384)
     ap.add_argument("--enc-predict", type=int, default=500_000)
     ap.add_argument("--data", help=argparse.SUPPRESS)
@@ -582,6 +654,9 @@ def main():
         return
     if args.encoder_trials:
         print(json.dumps(encoder_trials_measure()), flush=True)
+        return
+    if args.ind_trials:
+        print(json.dumps(ind_trials_measure()), flush=True)
         return
     if args.copy_calibration:
         import torch
