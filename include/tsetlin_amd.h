@@ -390,8 +390,9 @@ int tm_batch_train_encoder_dev(tm_batch *b, uint32_t n, const uint8_t *sentences
  * tm_create_classifier_ind / tm_create_encoder_ind), and training takes the
  * problems' byte size and dim_offsets as tm_train_classifier_ind /
  * tm_train_encoder_ind do.  One launch per epoch trains every class of every
- * trial; each trial equals its solo individualized machine.  These batches
- * are scored through tm_batch_machine (the *_ind predict calls). */
+ * trial; each trial equals its solo individualized machine.  Classifier
+ * batches are scored by tm_batch_class_accuracy_ind_dev, encoder batches
+ * through tm_batch_machine (tm_predict_encoder_ind_dev). */
 int tm_batch_create_classifier_ind(const tm_options *opts, uint32_t ntrials, const int64_t *feat_offsets,
                                    uint32_t n_offsets, tm_batch **out);
 int tm_batch_create_encoder_ind(const tm_options *opts, uint32_t ntrials, const int64_t *feat_offsets,
@@ -408,6 +409,12 @@ int tm_batch_train_encoder_ind(tm_batch *b, uint32_t n, const uint8_t *sentences
 int tm_batch_train_encoder_ind_dev(tm_batch *b, uint32_t n, const uint8_t *sentences_dev, size_t sentences_bytes,
                                    const uint8_t *codes_dev, const int64_t *dim_offsets, uint32_t iterations,
                                    tm_batch_epoch_fn each, void *user);
+/* tm_batch_class_accuracy_dev for a batch of individualized classifiers: the
+ * dev problems buffer read at dim_offsets (as tm_predict_classifier_ind_dev),
+ * macro[3 t .. 3 t + 2] of trial t; one synchronisation for all trials. */
+int tm_batch_class_accuracy_ind_dev(tm_batch *b, const uint8_t *problems_dev, size_t problems_bytes,
+                                    const int64_t *solutions_dev, const int64_t *dim_offsets, uint32_t n,
+                                    double *macro);
 /* Held-out accuracy of every trial on device-resident rows and labels. */
 int tm_batch_accuracy_dev(tm_batch *b, const uint8_t *problems_dev, const int64_t *solutions_dev, uint32_t n,
                           double *accuracy);

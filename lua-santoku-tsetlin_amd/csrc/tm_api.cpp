@@ -1577,6 +1577,7 @@ struct tm_batch {
   dbuf trials, block_map, perm, perm_scratch, ekeys, epoch, stats, rows, words, labels, codes, hits, pred;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<hipEvent_t> tev;   // individualized scoring: trial t's predictions are ready
   uint32_t lanes = 0, wpl = 0, order = 0;
   double tuned_ms = 0, train_ms = 0, eval_ms = 0;
 };
@@ -2093,7 +2094,7 @@ int tm_batch_class_accuracy_dev(tm_batch *b, const uint8_t *problems, const int6
 {
   if (!b || !macro) return fail(TM_E_ARG, "batch class accuracy: NULL argument");
   if (b->type != TM_CLASSIFIER) return fail(TM_E_ARG, "batch class accuracy: the batch holds encoder trials");
-  if (b->ind) return fail(TM_E_ARG, "batch class accuracy: individualized trials are scored through tm_batch_machine");
+  if (b->ind) return fail(TM_E_ARG, "batch class accuracy: individualized trials are scored by tm_batch_class_accuracy_ind_dev");
   const uint32_t nt = (uint32_t) b->m.size(), K = b->m[0]->classes;
   if (n == 0) {
     for (size_t i = 0; i < (size_t) 3 * nt; i++) macro[i] = 0;
@@ -2111,6 +2112,53 @@ int tm_batch_class_accuracy_dev(tm_batch *b, const uint8_t *problems, const int6
     if ((rc = predict_enqueue_dev(b->m[t], problems, n, b->pred.as<int64_t>()))) return rc;
     HIP_TRY(tm_launch_class_counts(b->pred.as<int64_t>(), solutions, n, K,
                                    b->hits.as<unsigned long long>() + t * per, b->stream));
+  }
+  HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  std::vector<uint64_t> h((size_t) nt * per);
+  HIP_TRY(hipMemcpyAsync(h.data(), b->hits.p, h.size() * 8, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->eval_ms = ms;
+  for (uint32_t t = 0; t < nt; t++) tm_finish_class_accuracy(&h[(size_t) t * per], K, nullptr, macro + 3 * t);
+  return TM_OK;
+}
+
+int tm_batch_class_accuracy_ind_dev(tm_batch *b, const uint8_t *problems, size_t problems_bytes,
+                                    const int64_t *solutions, const int64_t *dim_offsets, uint32_t n, double *macro)
+{
+  if (!b || !macro) return fail(TM_E_ARG, "batch class accuracy: NULL argument");
+  if (b->type != TM_CLASSIFIER) return fail(TM_E_ARG, "batch class accuracy: the batch holds encoder trials");
+  if (!b->ind) return fail(TM_E_ARG, "batch class accuracy: the batch is not individualized");
+  const uint32_t nt = (uint32_t) b->m.size(), K = b->m[0]->classes;
+  if (n == 0) {
+    for (size_t i = 0; i < (size_t) 3 * nt; i++) macro[i] = 0;
+    return TM_OK;
+  }
+  if (!problems || !solutions || !dim_offsets)
+    return fail(TM_E_ARG, "batch class accuracy: problems, solutions and dim_offsets must not be NULL");
+  HIP_TRY(hipSetDevice(b->device));
+  int rc;
+  const size_t per = (size_t) 3 * K;
+  if ((rc = b->hits.ensure((size_t) nt * per * 8))) return rc;
+  if ((rc = b->pred.ensure((size_t) nt * n * 8))) return rc;
+  while (b->tev.size() < nt) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->tev.push_back(e);
+  }
+  HIP_TRY(tm_launch_zero(b->hits.p, (size_t) nt * per * 8, b->stream));
+  HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  // each trial predicts on its own stream into its own slot; the counts on
+  // the batch's stream wait for it: one host synchronisation for all trials
+  for (uint32_t t = 0; t < nt; t++) {
+    tm_machine *m = b->m[t];
+    int64_t *pred = b->pred.as<int64_t>() + (size_t) t * n;
+    if ((rc = predict_sums_ind(m, problems, problems_bytes, dim_offsets, n, true))) return rc;
+    HIP_TRY(tm_launch_argmax(m->p_sums.as<int64_t>(), n, K, pred, m->stream));
+    HIP_TRY(hipEventRecord(b->tev[t], m->stream));
+    HIP_TRY(hipStreamWaitEvent(b->stream, b->tev[t], 0));
+    HIP_TRY(tm_launch_class_counts(pred, solutions, n, K, b->hits.as<unsigned long long>() + t * per, b->stream));
   }
   HIP_TRY(hipEventRecord(b->ev1, b->stream));
   std::vector<uint64_t> h((size_t) nt * per);
@@ -2147,6 +2195,7 @@ void tm_batch_destroy(tm_batch *b)
   if (b->stream) hipStreamSynchronize(b->stream);
   batch_release_graph(b);
   for (tm_machine *m : b->m) tm_destroy(m);
+  for (hipEvent_t e : b->tev) hipEventDestroy(e);
   dbuf *bufs[] = { &b->trials, &b->block_map, &b->perm, &b->perm_scratch, &b->ekeys, &b->epoch, &b->stats,
                    &b->rows, &b->words, &b->labels, &b->hits, &b->pred };
   for (dbuf *x : bufs) x->release();

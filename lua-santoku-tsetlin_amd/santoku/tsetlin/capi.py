@@ -35,6 +35,7 @@ EXPORTS = [
     "tm_batch_create_encoder", "tm_batch_train_encoder", "tm_batch_train_encoder_dev",
     "tm_batch_create_classifier_ind", "tm_batch_create_encoder_ind", "tm_batch_train_classifier_ind",
     "tm_batch_train_classifier_ind_dev", "tm_batch_train_encoder_ind", "tm_batch_train_encoder_ind_dev",
+    "tm_batch_class_accuracy_ind_dev",
     "tm_batch_accuracy_dev", "tm_batch_class_accuracy_dev", "tm_batch_tuning", "tm_batch_timing", "tm_batch_destroy",
 ]
 
@@ -153,6 +154,8 @@ def load_library(path=LIB_PATH):
                                                       BATCH_EPOCH_FN, vp]),
         "tm_batch_train_encoder_ind_dev": (ctypes.c_int, [vp, u32, u8p, ctypes.c_size_t, u8p, i64p, u32,
                                                           BATCH_EPOCH_FN, vp]),
+        "tm_batch_class_accuracy_ind_dev": (ctypes.c_int, [vp, u8p, ctypes.c_size_t, i64p, i64p, u32,
+                                                           ctypes.POINTER(ctypes.c_double)]),
         "tm_batch_accuracy_dev": (ctypes.c_int, [vp, u8p, i64p, u32, ctypes.POINTER(ctypes.c_double)]),
         "tm_batch_class_accuracy_dev": (ctypes.c_int, [vp, u8p, i64p, u32, ctypes.POINTER(ctypes.c_double)]),
         "tm_batch_tuning": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),

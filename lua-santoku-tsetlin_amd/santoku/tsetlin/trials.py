@@ -128,10 +128,21 @@ class Batch:
         _check(self._lib.tm_batch_accuracy_dev(self._h, pp, lp, int(n), out))
         return np.array(out[:], dtype=np.float64)
 
-    def class_accuracy(self, problems_dev, solutions_dev, n):
+    def class_accuracy(self, problems_dev, solutions_dev, n, dim_offsets=None):
         """evaluator.class_accuracy of every trial (tm_batch_class_accuracy_dev):
-        an (ntrials, 3) array of macro precision, recall, F1."""
+        an (ntrials, 3) array of macro precision, recall, F1.  Individualized
+        trials take the dev problems buffer and its dim_offsets
+        (tm_batch_class_accuracy_ind_dev)."""
         out = (ctypes.c_double * (3 * self.n))()
+        if self.individualized:
+            if dim_offsets is None:
+                raise TsetlinError("batch class accuracy: dim_offsets is required for individualized trials")
+            da, dp = _host_buffer(dim_offsets, np.int64)
+            problems_dev = problems_dev.contiguous()
+            pp = _device_buffer(problems_dev, np.uint8, 0, "batch class accuracy: problems")
+            lp = _device_buffer(solutions_dev, np.int64, int(n) * 8, "batch class accuracy: solutions")
+            _check(self._lib.tm_batch_class_accuracy_ind_dev(self._h, pp, problems_dev.numel(), lp, dp, int(n), out))
+            return np.array(out[:], dtype=np.float64).reshape(self.n, 3)
         row = self._machines[0].info()["input_chunks"]
         pp = _device_buffer(problems_dev, np.uint8, int(n) * row, "batch class accuracy: problems")
         lp = _device_buffer(solutions_dev, np.int64, int(n) * 8, "batch class accuracy: solutions")
@@ -202,15 +213,9 @@ def batch_trial_fn(base, train_rows, train_y, dev_rows, dev_y, iterations, patie
             return np.array([evaluator.encoding_accuracy(predict(b.machine(t)), dev_y, n_dev,
                                                          base["classes"])["mean_hamming"] for t in range(b.n)])
         if ind:
-            col = 2 if metric != "accuracy" else None
-            out = []
-            for t in range(b.n):
-                pred = predict(b.machine(t))
-                if col is None:
-                    out.append(float((pred == dev_y).sum()) / n_dev)
-                else:
-                    out.append(evaluator.class_accuracy(pred, dev_y, n_dev, base["classes"])["f1"])
-            return np.array(out)
+            if metric != "accuracy":
+                return b.class_accuracy(dev_rows, dev_y, n_dev, dim_offsets=dev_dim_offsets)[:, 2]
+            return np.array([float((predict(b.machine(t)) == dev_y).sum()) / n_dev for t in range(b.n)])
         if metric == "accuracy":
             return b.accuracy(dev_rows, dev_y, dev_y.shape[0])
         return b.class_accuracy(dev_rows, dev_y, dev_y.shape[0])[:, 2]

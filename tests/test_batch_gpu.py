@@ -387,3 +387,25 @@ def test_individualized_trial_fn_restores_best_epoch():
         assert np.array_equal(fn.last_batch.machine(t).checkpoint(), m.checkpoint()), (t, e)
         want = EV.class_accuracy(m.predict(dprob, doffs, nd), dy, nd, K)["f1"]
         assert scores[t] == want, (t, scores[t], want)
+
+
+def test_individualized_batch_class_accuracy_is_the_evaluator():
+    """tm_batch_class_accuracy_ind_dev: every trial's macro precision, recall
+    and F1 on an individualized dev buffer equal the evaluator on that
+    trial's own individualized predictions."""
+    import torch
+    from santoku.tsetlin import evaluator as EV
+    rng = np.random.default_rng(44)
+    widths, n, nd = [20, 45, 8, 90], 300, 220
+    K = len(widths)
+    prob, offs = _ragged(widths, n, rng)
+    dprob, doffs = _ragged(widths, nd, rng)
+    y = rng.integers(0, K, n).astype(np.int64)
+    dy = rng.integers(0, K, nd).astype(np.int64)
+    feat = np.concatenate([[0], np.cumsum(widths)]).astype(np.int64)
+    b = TR.Batch(dict(classes=K, features=1), trial_opts(np.random.default_rng(45), 5), feat_offsets=feat)
+    b.train(n, prob, y, 2, dim_offsets=offs)
+    got = b.class_accuracy(torch.from_numpy(dprob).cuda(), torch.from_numpy(dy).cuda(), nd, dim_offsets=doffs)
+    for t in range(5):
+        ev = EV.class_accuracy(b.machine(t).predict(dprob, doffs, nd), dy, nd, K)
+        assert (got[t, 0], got[t, 1], got[t, 2]) == (ev["precision"], ev["recall"], ev["f1"]), t
