@@ -409,3 +409,34 @@ def test_individualized_batch_class_accuracy_is_the_evaluator():
     for t in range(5):
         ev = EV.class_accuracy(b.machine(t).predict(dprob, doffs, nd), dy, nd, K)
         assert (got[t, 0], got[t, 1], got[t, 2]) == (ev["precision"], ev["recall"], ev["f1"]), t
+
+
+def test_individualized_encoder_trial_fn():
+    """The search's batch function over individualized encoder trials: each
+    result is mean_hamming of the restored best epoch's codes, equal to the
+    evaluator on a solo individualized encoder trained to that epoch."""
+    import torch
+    from santoku.tsetlin import evaluator as EV
+    from santoku.tsetlin import optimize as opt
+    rng = np.random.default_rng(47)
+    widths, n, nd = [33, 7, 120, 64, 15, 40, 9, 70, 22], 260, 120
+    K = len(widths)
+    prob, offs = _ragged(widths, n, rng)
+    dprob, doffs = _ragged(widths, nd, rng)
+    codes = rng.integers(0, 256, (n, (K + 7) // 8), dtype=np.uint8)
+    dcodes = rng.integers(0, 256, (nd, (K + 7) // 8), dtype=np.uint8)
+    feat = np.concatenate([[0], np.cumsum(widths)]).astype(np.int64)
+    base = dict(classes=K, features=1)
+    params = [{k: v for k, v in t.items() if k != "seed"} for t in trial_opts(np.random.default_rng(48), 3)]
+    fn = TR.batch_trial_fn(base, torch.from_numpy(prob).cuda(), torch.from_numpy(codes).cuda(),
+                           torch.from_numpy(dprob).cuda(), torch.from_numpy(dcodes).cuda(), iterations=5, patience=2,
+                           seed=700, encoder=True, feat_offsets=feat, dim_offsets=offs, dev_dim_offsets=doffs)
+    scores = fn(params, {"round": 1})
+    for t in range(3):
+        e = int(fn.last_best_epoch[t])
+        o = dict(opt._swap_tolerance(dict(params[t])), seed=700 + t)
+        m = tm.create("encoder", dict(base, reusable=True, individualized=True, feat_offsets=feat, **o))
+        m.train(samples=n, sentences=prob, codes=codes, dim_offsets=offs, iterations=e)
+        assert np.array_equal(fn.last_batch.machine(t).checkpoint(), m.checkpoint()), (t, e)
+        want = EV.encoding_accuracy(m.predict(dprob, doffs, nd), dcodes, nd, K)["mean_hamming"]
+        assert scores[t] == want, (t, scores[t], want)
